@@ -1,0 +1,165 @@
+/*
+ * periodsearch_amd.h -- C ABI of the MI355X (gfx950) convex-inversion period scan.
+ *
+ * The drop-in boundary for the hot path of Asteroids@home period_search: for every trial period x
+ * initial pole, the Levenberg-Marquardt fit of shape, spin and scattering parameters runs on the GPU
+ * (HIP kernels for gfx950); the host reads period_search_in, calls this ABI and writes
+ * period_search_out lines in the reference format.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository root):
+ *   ps_input_read/ps_input_parse  -> period_search/period_search_BOINC.cpp:277-485 (input parsing)
+ *   ps_geometry_build             -> period_search_BOINC.cpp:498-556 + trifac.c, areanorm.c,
+ *                                    sphfunc.c, ellfit.c (setup tables, initial ellipsoid)
+ *   ps_prepare                    -> CUDAPrepare, period_search_cuda/period_search/start_CUDA.h:2-4
+ *   ps_conw_search/ps_conw_round  -> CUDAPrecalc, start_CUDA.h:9-10 (period_search_BOINC.cpp:487-715)
+ *   ps_scan                       -> CUDAStart, start_CUDA.h:6-7, without the MFILE (the host writes)
+ *   ps_fit_batch                  -> the per-(period, pole) body of the scan, period_search_BOINC.cpp:817-921
+ *   ps_eval                       -> mrqcof (period_search/mrqcof.c:16-148) with bright()'s dyda
+ *   ps_release                    -> CUDAUnprepare, start_CUDA.h:14
+ *   ps_format_line                -> the out.printf of period_search_BOINC.cpp:927-931
+ *   ps_run_files                  -> the process contract of period_search_BOINC.cpp main (:147-954)
+ *
+ * Conventions: plain C types only; the caller owns every host array (read-only during a call); the
+ * library owns device memory.  Status codes: PS_OK (0) or a negative PS_ERR_*; ps_last_error()
+ * returns a message for the last failure on the calling thread.  Unlike the reference, no function
+ * calls exit().  Parameter vectors are 0-based: a[k] = cg[k+1] of the reference.
+ */
+#ifndef PERIODSEARCH_AMD_H
+#define PERIODSEARCH_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS_OK 0
+#define PS_ERR_ARG (-1)         /* invalid argument */
+#define PS_ERR_PARSE (-2)       /* malformed period_search_in */
+#define PS_ERR_LIMIT (-3)       /* a reference limit (constants.h) is exceeded */
+#define PS_ERR_DEVICE (-4)      /* HIP runtime failure / no device */
+#define PS_ERR_UNSUPPORTED (-5) /* shape not supported by the GPU kernels (see ps_prepare) */
+#define PS_ERR_NOMEM (-6)
+#define PS_ERR_IO (-7)
+
+#define PS_N_POLES 10 /* period_search/constants.h:11 */
+
+/* Precision of the fit kernels. */
+#define PS_FP64 0  /* FP64 throughout (the reference arithmetic) */
+#define PS_MIXED 1 /* FP32 facet sums / normal matrix, FP64 phase, chi2 test and solve */
+
+/* Parsed period_search_in.  Arrays are allocated by the library (ps_input_read/parse) or by the
+   caller (when the caller fills the struct itself); ps_input_free frees library-allocated ones. */
+typedef struct ps_input {
+    double per_start, per_step_coef, per_end; /* period interval [h] and step coefficient */
+    int ia_prd;                               /* period free (1) / fixed (0) */
+    double jd_00;                             /* epoch of zero time as read */
+    double phi0;                              /* initial rotation angle [rad] */
+    double conw;                              /* convexity weight */
+    int lmax, mmax, nrows;                    /* Laplace series degree/order, triangulation rows */
+    double par[3];                            /* phase-function initial values */
+    int ia_par[3];
+    double cl;                                /* initial Lambert coefficient */
+    int ia_cl;
+    double stop_condition;                    /* >1: iterations; <1: minimum rms improvement */
+    int n_iter_min;
+    double alamda_incr, alamda_start;
+    int lcurves;                              /* observed lightcurves */
+    int *lpoints;                             /* [lcurves] */
+    int *inrel;                               /* [lcurves] 1 = relative, 0 = calibrated */
+    int ndata;                                /* observed points */
+    double *tim;                              /* [ndata] JD - jd_0 */
+    double *brightness;                       /* [ndata] */
+    double *sig;                              /* [ndata] mean brightness of the point's lightcurve */
+    double *weight;                           /* [ndata] */
+    double *ee;                               /* [ndata*3] unit asteroid->Earth vectors */
+    double *ee0;                              /* [ndata*3] unit asteroid->Sun vectors */
+    double jd_min, jd_max, jd_0;
+    int onlyrel;                              /* 1 if every lightcurve is relative */
+    double escl;                              /* initial size scale */
+    double a_axis, b_axis, c_axis;            /* initial ellipsoid */
+    int owned;                                /* 1 if the arrays belong to the library */
+} ps_input;
+
+/* Setup tables for one (nrows, lmax, mmax): Gaussian-image triangulation and spherical harmonics. */
+typedef struct ps_geometry {
+    int nrows, nfac, ndir, lmax, mmax, ncoef, npairs;
+    double *nor;      /* [nfac*3] facet normals */
+    double *darea;    /* [nfac] facet areas on the unit sphere */
+    double *dsph;     /* [nfac*ncoef] spherical-harmonic basis per facet and coefficient */
+    double *pleg;     /* [nfac*npairs] associated Legendre values per (m, l) pair, curv order */
+    double *fc, *fs;  /* [nfac*(mmax+1)] cos(m phi), sin(m phi) */
+    int *pair_m, *pair_l; /* [npairs] */
+    double *cg_first; /* [ncoef] initial ellipsoid coefficients (ellfit) */
+} ps_geometry;
+
+/* One output line: 24*P [h], rms, chi2, dark facet % (conw*escl^2 on line 1), lambda, beta. */
+typedef struct ps_line {
+    double per, dev, chi2, dark, la, be;
+} ps_line;
+
+/* Result of one (period, pole) fit. */
+typedef struct ps_fit_result {
+    double dev;      /* rms of the best improvement, sqrt(rchisq / ndata) */
+    double prd;      /* fitted period [d] */
+    double dark;     /* |sum Area*n| at the last improvement */
+    double totarea;  /* sum Area of the last curv evaluation */
+    double la, be;   /* fitted pole [deg] (lambda not wrapped) */
+    double chisq;    /* chi2 of the last iteration */
+    int niter;       /* LM iterations run */
+    int nsolve_fail; /* Gauss-Jordan failures (the step was skipped) */
+} ps_fit_result;
+
+typedef struct ps_handle ps_handle;
+
+typedef struct ps_info {
+    int ndata, npts, nlc, nfac, ncoef, ma, mfit, nmax;
+    double freq_start, freq_end, freq_step, escl, conw_r0;
+    int slots;         /* fits resident on the device at once */
+    int precision;
+} ps_info;
+
+const char *ps_last_error(void);
+const char *ps_version(void);
+
+/* ---- input and setup (host) ---- */
+int ps_input_read(const char *path, ps_input **out);
+int ps_input_parse(const char *text, ps_input **out);
+void ps_input_free(ps_input *in);
+int ps_geometry_build(const ps_input *in, ps_geometry **out);
+void ps_geometry_free(ps_geometry *geo);
+
+/* ---- device ---- */
+/* Uploads points, lightcurve table, facet tables and the free/fixed mask; allocates per-slot scratch.
+   Returns PS_ERR_UNSUPPORTED when mfit > 64 or nfac > 1024 (lmax <= 6 is supported). */
+int ps_prepare(const ps_input *in, const ps_geometry *geo, int device, ps_handle **out);
+int ps_set_precision(ps_handle *h, int precision);
+int ps_get_info(const ps_handle *h, ps_info *info);
+/* Runs nfits independent fits (freq[i] in 1/h, pole[i] in 1..10) with convexity weight conw_r. */
+int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r, ps_fit_result *out);
+/* One round of the convexity-weight search over the first min(10, nmax) periods; ave_dark = mean
+   best-pole dark-facet % (NaN counted as 1.0). */
+int ps_conw_round(ps_handle *h, double conw_r, double *ave_dark);
+/* Full convexity-weight search starting from conw/escl^2; returns new_conw (1 = converged). */
+int ps_conw_search(ps_handle *h, double *conw_r, int *new_conw);
+/* Scans periods n_from..n_to (1-based, inclusive; n_to <= 0 means nmax) into lines[n_to-n_from+1]. */
+int ps_scan(ps_handle *h, int n_from, int n_to, double conw_r, ps_line *lines);
+/* mrqcof for nvec parameter vectors a[nvec*ma]: alpha[nvec*mfit*mfit], beta[nvec*mfit], chisq[nvec]
+   (normalised normal equations, fitted parameters only); when ymod/dyda are non-NULL also the raw
+   bright()/conv() outputs per point, ymod[nvec*npts], dyda[nvec*npts*ma], before lightcurve
+   normalisation.  Any output pointer may be NULL. */
+int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, double *alpha, double *beta, double *chisq,
+            double *ymod, double *dyda);
+/* Device time of the fit kernels since the last reset, measured with HIP events on the library's
+   stream: total milliseconds and number of launches. */
+int ps_kernel_time(ps_handle *h, double *ms, int *launches, int reset);
+void ps_release(ps_handle *h);
+
+/* ---- output and process contract (host) ---- */
+int ps_format_line(const ps_line *line, char *buf, int buflen);
+/* period_search_in -> period_search_out with checkpoint/resume through state_path (reference state
+   format: the number of lines written).  Lines are appended when a valid state file exists. */
+int ps_run_files(const char *in_path, const char *out_path, const char *state_path, int device, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
