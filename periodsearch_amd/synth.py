@@ -176,7 +176,9 @@ def generate(spec: SynthSpec):
 def config0(**kw):
     """Plain reference method, FP64, CPU-runnable: 10 dense lightcurves x 60 points over 2 years,
     Lmax = 3 fit on 288 facets, scan 6.9-7.8 h."""
-    return SynthSpec(seed=1000, **kw)
+    base = dict(seed=1000)
+    base.update(kw)
+    return SynthSpec(**base)
 
 
 def config1(**kw):
