@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Benchmark of the period scan hot path: trial-period Levenberg-Marquardt fits per second at
+Lmax = Mmax = 6 on 288 facets (the production shape: 40 dense lightcurves x 50 points + 400 sparse
+calibrated points, ndata = 2400, 55 fitted parameters, stop condition 50 = fixed 50 LM iterations).
+
+A step is one ps_fit_batch call over a window of trial periods x 10 initial poles, inputs resident
+in HBM (uploaded once by ps_prepare); ps_fit_batch returns after the library's stream has
+synchronised, so the wall clock around the K steps covers all device work.  The roofline figure uses
+the fit kernel's own duration, measured with HIP events recorded on the stream it is launched on.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--periods P]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "trial-period LM fits/sec at Lmax=6, 288 facets: mixed vs FP64, same best period"
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
+ORACLE_CLI = os.path.join(ROOT, "oracle", "ps_oracle_cli")
+
+
+def algorithmic_flops_per_eval(info):
+    """Shape-Jacobian contraction + J^T W J lower triangle with diagonal + J^T W r, per model
+    evaluation: N * (2 Nf Nc + mfit (mfit + 1) + 2 mfit), N = ndata + 3 convexity rows."""
+    n, nf, nc, mf = info["npts"], info["nfac"], info["ncoef"], info["mfit"]
+    return n * (2 * nf * nc + mf * (mf + 1) + 2 * mf)
+
+
+def traffic_per_launch(workload, precision):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(input_path, conw_r, info, seconds_target=20.0):
+    """The oracle (FP64 CPU restatement, one fit per process at a time) on the host's cores: each
+    process runs all 10 poles of one trial period, or fewer poles to stay near seconds_target."""
+    if not os.path.exists(ORACLE_CLI):
+        return None
+    cores = max(1, min(16, os.cpu_count() or 1))
+    probe = subprocess.run([ORACLE_CLI, "fits", input_path, "1", "1", repr(conw_r), "1"], capture_output=True,
+                           text=True, check=True)
+    t_fit = float(probe.stdout.split()[3])
+    npoles = max(1, min(10, int(seconds_target / max(t_fit, 1e-3))))
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([ORACLE_CLI, "fits", input_path, str(2 + k), str(2 + k), repr(conw_r), str(npoles)],
+                              stdout=subprocess.PIPE, text=True) for k in range(cores)]
+    nfits = 0
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            return None
+        nfits += int(out.split()[1])
+    wall = time.perf_counter() - t0
+    return {"value": nfits / wall, "unit": "fits/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} processes x 1 trial period x {npoles} poles of the same workload "
+                      f"({nfits} fits, {wall:.1f} s wall), oracle/ps_oracle_cli FP64 restatement"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the scan shards by "
+              "independent period ranges, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from periodsearch_amd import PS_FP64, PS_MIXED, PeriodSearch, synth
+
+    text, _ = synth.generate(synth.config1())
+    tmpdir = tempfile.mkdtemp(prefix="psbench_")
+    input_path = os.path.join(tmpdir, "period_search_in")
+    with open(input_path, "w") as f:
+        f.write(text)
+    ps = PeriodSearch.from_file(input_path).prepare(device=0,
+                                                    precision=PS_MIXED if args.precision == "mixed" else PS_FP64)
+    info = ps.info()
+    conw_r = info["conw_r0"]
+    per_step = args.periods * 10
+
+    def step(k):
+        n = 1 + (k * args.periods + np.arange(per_step) // 10) % info["nmax"]
+        pole = 1 + np.arange(per_step) % 10
+        return ps.fit_batch(ps.freq(n), pole, conw_r)
+
+    for k in range(args.warmup):
+        step(k)
+    ps.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res = step(args.warmup + k)
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = ps.kernel_time()
+    fits = per_step * args.steps
+    value = fits / wall
+
+    niter = float(np.mean(res["niter"]))
+    f_fit = (niter + 1) * algorithmic_flops_per_eval(info)
+    avg_launch_s = kernel_ms * 1e-3 / launches
+    achieved = per_step * f_fit / avg_launch_s / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    workload = "config1_lmax6_288fac_ndata2400"
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded generator, periodsearch_amd/synth.py config1)",
+        "config": {"workload": workload, "ndata": info["ndata"], "lmax": 6, "nfac": info["nfac"],
+                   "mfit": info["mfit"], "lm_iterations": niter, "periods_per_step": args.periods,
+                   "fits_per_step": per_step, "precision": args.precision,
+                   "trial_periods_per_s": round(value / 10, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4),
+                     "traffic": traffic_per_launch(workload, args.precision),
+                     "flops_per_fit": f_fit, "avg_launch_ms": round(avg_launch_s * 1e3, 3)},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(input_path, conw_r, info),
+    }
+    ps.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

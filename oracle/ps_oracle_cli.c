@@ -4,9 +4,9 @@
  *   ps_oracle_cli scan  <period_search_in> <n_from> <n_to> [conw_r]
  *       prints period_search_out lines (reference format) for periods n_from..n_to; when conw_r
  *       is omitted the convexity-weight search runs first (period_search_BOINC.cpp:487-715).
- *   ps_oracle_cli fits  <period_search_in> <n_from> <n_to> <conw_r>
- *       runs the (period, pole) fits of periods n_from..n_to and prints one line with the fit
- *       count and the wall time in seconds (the CPU-baseline timing mode).
+ *   ps_oracle_cli fits  <period_search_in> <n_from> <n_to> <conw_r> [npoles]
+ *       runs the (period, pole) fits of periods n_from..n_to (poles 1..npoles, default 10) and
+ *       prints one line with the fit count and the wall time in seconds (CPU-baseline timing).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -23,7 +23,7 @@ static double now(void) {
 
 int main(int argc, char **argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s scan|fits <period_search_in> <n_from> <n_to> [conw_r]\n", argv[0]);
+        fprintf(stderr, "usage: %s scan|fits <period_search_in> <n_from> <n_to> [conw_r [npoles]]\n", argv[0]);
         return 2;
     }
     int err = 0;
@@ -42,10 +42,12 @@ int main(int argc, char **argv) {
         double f0, df;
         pso_ctx_freq(c, &f0, NULL, &df, NULL);
         pso_ctx_set_conw(c, conw_r);
+        int npoles = argc > 6 ? atoi(argv[6]) : PSO_N_POLES;
+        if (npoles < 1 || npoles > PSO_N_POLES) npoles = PSO_N_POLES;
         double t0 = now();
         int nfits = 0;
         for (int n = n_from; n <= n_to; n++)
-            for (int m = 1; m <= PSO_N_POLES; m++) {
+            for (int m = 1; m <= npoles; m++) {
                 pso_fit_result r;
                 pso_fit(c, f0 - (n - 1) * df, m, &r);
                 nfits++;

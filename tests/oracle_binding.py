@@ -177,6 +177,14 @@ class OracleCtx:
         chi = self.o.pso_mrqcof(self.c, _p(cg), ia, self.ma, _p(al), _p(be))
         return al.reshape(self.mfit + 1, self.mfit + 1)[1:, 1:].copy(), be[1:self.mfit + 1].copy(), chi
 
+    def conv(self, a, comp):
+        """conv() (value, dres[ma]) for convexity component comp in 1..3 after curv of a."""
+        cg = np.concatenate([[0.0], a])
+        self.o.pso_curv(self.c, _p(cg))
+        dres = np.zeros(self.ma + 1)
+        res = self.o.pso_conv(self.c, comp, _p(dres), self.ma)
+        return res, dres[1:].copy()
+
     def bright(self, a, ee, ee0, t):
         """bright() after curv/blmatrix of a: (brightness, dyda[ma])."""
         cg = np.concatenate([[0.0], a])
