@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--compare", action="store_true",
+                    help="also run the other precision on the same fits: speedup and best-period agreement")
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

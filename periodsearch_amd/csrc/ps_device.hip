@@ -40,7 +40,7 @@ T *dev_upload(const T *h, size_t n, std::vector<void *> &owned) {
 // Coefficient layout of the shape Jacobian: NT full 16-column MFMA tiles + R VALU columns.
 struct Variant {
     int nt, r;
-    bool lds;
+    bool lds, lds_mix;  // Dsph fits in LDS as FP64 / as FP32
 };
 Variant pick_variant(int nc, int nf) {
     Variant v;
@@ -52,6 +52,7 @@ Variant pick_variant(int nc, int nf) {
         default: v = {4, 0, true}; break;  // any nc <= 64, zero-padded to four tiles
     }
     v.lds = nf * (16 * v.nt + v.r) <= psk::kDsphLds;
+    v.lds_mix = nf * (16 * v.nt + v.r) <= 2 * psk::kDsphLds;
     return v;
 }
 }  // namespace
@@ -86,26 +87,30 @@ namespace {
         if (e_ != hipSuccess) return ps_fail(PS_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_));          \
     } while (0)
 
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 int launch_fit(ps_handle *h, int grid, int nfits, double conw_r) {
-    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, h->d_freq,
+    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, h->d_freq,
                        h->d_pole, nfits, conw_r, h->d_out);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 int launch_eval(ps_handle *h, int grid, const double *a, int nvec, double *al, double *be, double *ch, double *ry,
                 double *rd) {
-    hipLaunchKernelGGL((psk::eval_kernel<NT, R, LDSB>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, a, nvec, al,
-                       be, ch, ry, rd);
+    hipLaunchKernelGGL((psk::eval_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, a, nvec,
+                       al, be, ch, ry, rd);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
 
-// Calls f.template operator()<NT, R, LDSB>() for the handle's kernel variant.
+// Calls f.template operator()<NT, R, LDSB, MIX>() for the handle's kernel variant and precision.
 template <class F>
-int dispatch(const Variant &v, F &&f) {
-#define PS_VARIANT(NT_, R_)                                                      \
-    if (v.nt == NT_ && v.r == R_)                                                \
-        return v.lds ? f.template operator()<NT_, R_, true>() : f.template operator()<NT_, R_, false>();
+int dispatch(const Variant &v, bool mix, F &&f) {
+#define PS_VARIANT(NT_, R_)                                                                              \
+    if (v.nt == NT_ && v.r == R_) {                                                                      \
+        if (mix)                                                                                         \
+            return v.lds_mix ? f.template operator()<NT_, R_, true, true>()                              \
+                             : f.template operator()<NT_, R_, false, true>();                            \
+        return v.lds ? f.template operator()<NT_, R_, true, false>() : f.template operator()<NT_, R_, false, false>(); \
+    }
     PS_VARIANT(1, 0)
     PS_VARIANT(1, 9)
     PS_VARIANT(2, 4)
@@ -278,6 +283,8 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     for (int i = 0; i < nf; ++i)
         for (int k = 0; k < ncoef; ++k) dsph_pad[size_t(i) * P.dpitch + k] = geo->dsph[size_t(i) * ncoef + k];
     P.dsph = dev_upload(dsph_pad.data(), dsph_pad.size(), o);
+    std::vector<float> dsph_f(dsph_pad.begin(), dsph_pad.end());
+    P.dsphf = dev_upload(dsph_f.data(), dsph_f.size(), o);
     P.pleg = dev_upload(geo->pleg, size_t(nf) * geo->npairs, o);
     P.fc = dev_upload(geo->fc, size_t(nf) * (geo->mmax + 1), o);
     P.fs = dev_upload(geo->fs, size_t(nf) * (geo->mmax + 1), o);
@@ -293,8 +300,8 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(ps_fail(PS_ERR_DEVICE, "device query failed"));
     int per_cu = 1;
-    dispatch(h->var, [&]<int NT, int R, bool LDSB>() {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::fit_kernel<NT, R, LDSB>, kWG, 0);
+    dispatch(h->var, false, [&]<int NT, int R, bool LDSB, bool MIX>() {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::fit_kernel<NT, R, LDSB, MIX>, kWG, 0);
         return 0;
     });
     h->slots = std::max(1, per_cu) * prop.multiProcessorCount;
@@ -322,7 +329,8 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
 
 extern "C" int ps_set_precision(ps_handle *h, int precision) {
     if (!h) return ps_fail(PS_ERR_ARG, "ps_set_precision: null handle");
-    if (precision != PS_FP64) return ps_fail(PS_ERR_UNSUPPORTED, "precision %d not available", precision);
+    if (precision != PS_FP64 && precision != PS_MIXED)
+        return ps_fail(PS_ERR_ARG, "precision %d: use PS_FP64 or PS_MIXED", precision);
     h->precision = precision;
     return PS_OK;
 }
@@ -375,7 +383,9 @@ extern "C" int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const i
     HIPCHECK(hipMemcpyAsync(h->d_pole, pole, sizeof(int) * nfits, hipMemcpyHostToDevice, h->stream));
     const int grid = std::min(nfits, h->slots);
     HIPCHECK(hipEventRecord(h->ev0, h->stream));
-    rc = dispatch(h->var, [&]<int NT, int R, bool LDSB>() { return launch_fit<NT, R, LDSB>(h, grid, nfits, conw_r); });
+    rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
+        return launch_fit<NT, R, LDSB, MIX>(h, grid, nfits, conw_r);
+    });
     if (rc != PS_OK) return ps_fail(rc, "fit kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIPCHECK(hipEventRecord(h->ev1, h->stream));
     HIPCHECK(hipMemcpyAsync(out, h->d_out, sizeof(ps_fit_result) * nfits, hipMemcpyDeviceToHost, h->stream));
@@ -419,8 +429,8 @@ extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, d
     hipError_t e = hipMemcpyAsync(d_a, a, sizeof(double) * nvec * h->ma, hipMemcpyHostToDevice, h->stream);
     const int grid = std::min(nvec, h->slots);
     if (e == hipSuccess)
-        rc = dispatch(h->var, [&]<int NT, int R, bool LDSB>() {
-            return launch_eval<NT, R, LDSB>(h, grid, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
+        rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
+            return launch_eval<NT, R, LDSB, MIX>(h, grid, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
         });
     if (e == hipSuccess && rc == PS_OK && alpha)
         e = hipMemcpyAsync(alpha, d_al, sizeof(double) * nvec * mf2, hipMemcpyDeviceToHost, h->stream);

@@ -40,6 +40,7 @@ constexpr double kTiny = 1e-8;  // period_search/constants.h:10
 constexpr double kPi = M_PI;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 struct DevProblem {
     int npts, nobs, nlc, nf, nc, ma, mfit, npairs, mmax1, dpitch;
@@ -52,6 +53,7 @@ struct DevProblem {
     const int *lc_start, *lc_inrel, *lc_of;
     const double *nx, *ny, *nz, *darea;
     const double *dsph;  // [nf][dpitch], zero-padded beyond nc
+    const float *dsphf;  // the same in FP32 (mixed precision)
     const double *pleg, *fc, *fs;
     const int *pair_m;
     const int *fit_idx;
@@ -69,7 +71,10 @@ struct Shared {
     double cg[kMaxMa];    // accepted parameters
     double atry[kMaxMa];
     double nx[kMaxFac], ny[kMaxFac], nz[kMaxFac], area[kMaxFac], g[kMaxFac];
-    double da[64], col[64];
+    float nxf[kMaxFac], nyf[kMaxFac], nzf[kMaxFac], areaf[kMaxFac];  // FP32 copies (mixed precision)
+    double da[64], col[64], b0[64];
+    int indxr[64], indxc[64];
+    double chi_part[kWaves];
     double red_v[kWaves];
     int red_q[kWaves];
     int ipiv[64];
@@ -77,25 +82,47 @@ struct Shared {
     int iscal[4];
     double bl[9], dbl[18];  // pole rotation and its beta/lambda derivatives (blmatrix)
     union {
-        double dsph[kDsphLds];  // phase B
-        double cov[64 * 64];    // Gauss-Jordan
+        double dsph[kDsphLds];       // phase B, FP64
+        float dsphf[2 * kDsphLds];   // phase B, FP32
+        struct {
+            double cov[64 * 64];     // Gauss-Jordan: the damped matrix, then its inverse
+            double cov0[64 * 64];    // the damped matrix kept for iterative refinement
+        } g;
         struct {
             double stage[kWaves][4 * 64];
             double red[64 * 64];
-        } d;                    // phase D
+        } d;                         // phase D
     } u;
 };
 
-__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-
-// 1/x from v_rcp_f64 and two Newton steps (rounding within an ulp of the IEEE quotient).
-__device__ inline double fast_rcp(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
+// Precision traits of the two MFMA contractions.  FP64: v_mfma_f64_16x16x4f64, result row 4r + (l>>4)
+// in register r; FP32: v_mfma_f32_16x16x4f32, result row 4(l>>4) + r (lane layouts measured on gfx950).
+template <bool MIX>
+struct Prec;
+template <>
+struct Prec<false> {
+    using T = double;
+    using V4 = d4;
+    __device__ static V4 mma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int r, int kg) { return 4 * r + kg; }
+    __device__ static V4 zero() { return V4{0, 0, 0, 0}; }
+    __device__ static double rcp(double x) {  // v_rcp_f64 and two Newton steps: within an ulp of 1/x
+        double y = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+        e = fma(-x, y, 1.0);
+        return fma(y, e, y);
+    }
+};
+template <>
+struct Prec<true> {
+    using T = float;
+    using V4 = f4;
+    __device__ static V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int r, int kg) { return 4 * kg + r; }
+    __device__ static V4 zero() { return V4{0, 0, 0, 0}; }
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+};
 
 // ---------------------------------------------------------------------------------------------
 // curv (period_search/curv.c:10-39): g = exp(sum_{m,l} Pleg*(c*Fc + s*Fs)), Area = Darea*g.
@@ -221,7 +248,8 @@ __device__ inline void point_derivs(const DevProblem &P, const double *a, int p,
         }
 }
 
-__device__ inline double sum_groups(double v) {  // sum over the four 16-lane groups of a wave
+template <class T>
+__device__ inline T sum_groups(T v) {  // sum over the four 16-lane groups of a wave
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
     return v;
@@ -230,51 +258,65 @@ __device__ inline double sum_groups(double v) {  // sum over the four 16-lane gr
 // ---------------------------------------------------------------------------------------------
 // Phase B: bright (period_search/bright.c:14-126) for every observed point, conv (conv.c:12-29) for
 // the three convexity rows.  NT full 16-column MFMA tiles + R leftover coefficients (NT*16 + R >= nc).
+// MIX: facet terms, rotation sums and the shape contraction in FP32; the body-frame directions and
+// the brightness (the model value entering the residual) stay FP64.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 __device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp, bool all_derivs) {
+    using Pr = Prec<MIX>;
+    using T = typename Pr::T;
+    using V4 = typename Pr::V4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int pl = lane & 15, kg = lane >> 4;
     const int nc = P.nc, pitch = P.dpitch;
     const double *a = sh.a;
-    const double *dsph = LDSB ? sh.u.dsph : P.dsph;
-    const double cl = exp(a[nc + 6]);
-    const double cls = a[nc + 7];
+    const T *dsph, *snx, *sny, *snz, *sar;
+    if constexpr (MIX) {
+        dsph = LDSB ? sh.u.dsphf : P.dsphf;
+        snx = sh.nxf; sny = sh.nyf; snz = sh.nzf; sar = sh.areaf;
+    } else {
+        dsph = LDSB ? sh.u.dsph : P.dsph;
+        snx = sh.nx; sny = sh.ny; snz = sh.nz; sar = sh.area;
+    }
+    const double cl_d = exp(a[nc + 6]);
+    const T cl = T(cl_d), cls = T(a[nc + 7]);
     const bool want_d1 = P.ia_cl || all_derivs, want_d2 = P.ia_ls || all_derivs;
     const int ntiles = (P.nobs + 15) >> 4;
     const int nchunk = P.nf >> 2;
     for (int tile = wave; tile < ntiles; tile += kWaves) {
         const int p = tile * 16 + pl;
         const bool valid = p < P.nobs;
-        double e[3], e0[3];
-        point_dirs(P, a, valid ? p : 0, sh, e, e0);
-        d4 acc[NT];
+        double ed[3], e0d[3];
+        point_dirs(P, a, valid ? p : 0, sh, ed, e0d);
+        const T e[3] = {T(ed[0]), T(ed[1]), T(ed[2])}, e0[3] = {T(e0d[0]), T(e0d[1]), T(e0d[2])};
+        V4 acc[NT];
 #pragma unroll
-        for (int tt = 0; tt < NT; ++tt) acc[tt] = d4{0, 0, 0, 0};
-        double racc[R > 0 ? R : 1];
+        for (int tt = 0; tt < NT; ++tt) acc[tt] = Pr::zero();
+        T racc[R > 0 ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; ++r) racc[r] = 0;
-        double br = 0, S0 = 0, S1 = 0, S2 = 0, T0 = 0, T1 = 0, T2 = 0, d1 = 0, d2 = 0;
+        double br = 0;
+        T S0 = 0, S1 = 0, S2 = 0, T0 = 0, T1 = 0, T2 = 0, d1 = 0, d2 = 0;
         for (int c = 0; c < nchunk; ++c) {
             const int j = 4 * c + kg;
-            const double n0 = sh.nx[j], n1 = sh.ny[j], n2 = sh.nz[j], A = sh.area[j];
-            const double mu = e[0] * n0 + e[1] * n1 + e[2] * n2;
-            const double mu0 = e0[0] * n0 + e0[1] * n1 + e0[2] * n2;
-            const bool vis = valid && (mu > kTiny) && (mu0 > kTiny);
-            const double rd = fast_rcp(mu + mu0);
-            const double s = mu * mu0 * (cl + cls * rd);
-            const double X = vis ? A * s : 0.0;
-            br += X;
-            const double q0 = mu0 * rd, q = mu * rd;
-            const double ad = vis ? A * (cls * (q0 * q0) + cl * mu0) : 0.0;
-            const double ad0 = vis ? A * (cls * (q * q) + cl * mu) : 0.0;
+            const T n0 = snx[j], n1 = sny[j], n2 = snz[j], A = sar[j];
+            const T mu = e[0] * n0 + e[1] * n1 + e[2] * n2;
+            const T mu0 = e0[0] * n0 + e0[1] * n1 + e0[2] * n2;
+            const bool vis = valid && (mu > T(kTiny)) && (mu0 > T(kTiny));
+            const T rd = Pr::rcp(mu + mu0);
+            const T s = mu * mu0 * (cl + cls * rd);
+            const T X = vis ? A * s : T(0);
+            br += double(X);
+            const T q0 = mu0 * rd, q = mu * rd;
+            const T ad = vis ? A * (cls * (q0 * q0) + cl * mu0) : T(0);
+            const T ad0 = vis ? A * (cls * (q * q) + cl * mu) : T(0);
             S0 += ad * n0; S1 += ad * n1; S2 += ad * n2;
             T0 += ad0 * n0; T1 += ad0 * n1; T2 += ad0 * n2;
-            if (want_d1) d1 += vis ? mu * mu0 * A : 0.0;
-            if (want_d2) d2 += vis ? A * mu * mu0 * rd : 0.0;
-            const double *brow = dsph + j * pitch;
+            if (want_d1) d1 += vis ? mu * mu0 * A : T(0);
+            if (want_d2) d2 += vis ? A * mu * mu0 * rd : T(0);
+            const T *brow = dsph + j * pitch;
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) acc[tt] = mfma(X, brow[16 * tt + pl], acc[tt]);
+            for (int tt = 0; tt < NT; ++tt) acc[tt] = Pr::mma(X, brow[16 * tt + pl], acc[tt]);
 #pragma unroll
             for (int r = 0; r < R; ++r) racc[r] += X * brow[16 * NT + r];
         }
@@ -292,32 +334,33 @@ __device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *y
             double *jr = J + size_t(p) * kMap;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (16 * NT + r < nc) jr[16 * NT + r] = scale * racc[r];
+                if (16 * NT + r < nc) jr[16 * NT + r] = scale * double(racc[r]);
+            const double s0 = S0, s1 = S1, s2 = S2, t0 = T0, t1 = T1, t2 = T2;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                jr[nc + k] = scale * (S0 * G.de[0][k] + S1 * G.de[1][k] + S2 * G.de[2][k] +
-                                      (T0 * G.de0[0][k] + T1 * G.de0[1][k] + T2 * G.de0[2][k]));
+                jr[nc + k] = scale * (s0 * G.de[0][k] + s1 * G.de[1][k] + s2 * G.de[2][k] +
+                                      (t0 * G.de0[0][k] + t1 * G.de0[1][k] + t2 * G.de0[2][k]));
 #pragma unroll
             for (int i = 0; i < 3; ++i) jr[nc + 3 + i] = br * G.dphp[i];
-            jr[nc + 6] = scale * d1 * cl;
-            jr[nc + 7] = scale * d2;
+            jr[nc + 6] = scale * double(d1) * cl_d;
+            jr[nc + 7] = scale * double(d2);
             ytemp[p] = br * scale;
         }
-        // shape columns: register r of lane l holds point 4r + kg, coefficient 16tt + pl
+        // shape columns: register r of lane l holds point Pr::row(r, kg), coefficient 16tt + pl
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = 4 * r + kg;
+            const int i = Pr::row(r, kg);
             const double sc = __shfl(scale, i);
             const int pi = tile * 16 + i;
             if (pi < P.nobs) {
                 double *jr = J + size_t(pi) * kMap;
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt)
-                    if (16 * tt + pl < nc) jr[16 * tt + pl] = sc * acc[tt][r];
+                    if (16 * tt + pl < nc) jr[16 * tt + pl] = sc * double(acc[tt][r]);
             }
         }
     }
-    // convexity rows: res_c = sum Area*n_c, dres_k = sum Area*Dsph_k*n_c (conv.c; Area = Darea*g)
+    // convexity rows: res_c = sum Area*n_c, dres_k = sum Area*Dsph_k*n_c (conv.c; Area = Darea*g), FP64
     for (int it = threadIdx.x; it < 3 * (nc + 1); it += kWG) {
         const int c = it / (nc + 1), k = it - c * (nc + 1);
         const double *nn = (c == 0) ? sh.nx : (c == 1) ? sh.ny : sh.nz;
@@ -326,7 +369,7 @@ __device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *y
             for (int i = 0; i < P.nf; ++i) s += sh.area[i] * nn[i];
             ytemp[P.nobs + c] = s;
         } else {
-            for (int i = 0; i < P.nf; ++i) s += sh.area[i] * dsph[i * pitch + k] * nn[i];
+            for (int i = 0; i < P.nf; ++i) s += sh.area[i] * P.dsph[i * pitch + k] * nn[i];
             J[size_t(P.nobs + c) * kMap + k] = s;
         }
     }
@@ -339,19 +382,30 @@ __device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *y
 // ---------------------------------------------------------------------------------------------
 // mrqcof (period_search/mrqcof.c:16-148) for the parameters in sh.a: writes alpha[mfit][mfit]
 // (full, symmetric) and beta[mfit], returns chi2.  raw_y / raw_dyda receive the raw model rows.
+// MIX: the normal-equation contraction runs on FP32 MFMA with FP32 accumulation; chi2 (the LM
+// acceptance test) is summed in FP64 from the FP64 residuals.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *dave, double *alpha, double *beta,
                          Shared &sh, double *raw_y, double *raw_dyda) {
+    using Pr = Prec<MIX>;
+    using T = typename Pr::T;
+    using V4 = typename Pr::V4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int pl = lane & 15, kg = lane >> 4;
     const int ma = P.ma, npts = P.npts, mfit = P.mfit;
     curv_lds(P, sh.a, sh);
     blmatrix_lds(sh.a[P.nc], sh.a[P.nc + 1], sh);
-    if (LDSB)
-        for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = P.dsph[i];
+    if (LDSB) {
+        if constexpr (MIX)
+            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = P.dsphf[i];
+        else
+            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = P.dsph[i];
+    }
+    if constexpr (MIX)
+        for (int i = tid; i < P.nf; i += kWG) sh.areaf[i] = float(sh.area[i]);
     __syncthreads();
-    model_rows<NT, R, LDSB>(P, sh, J, ytemp, raw_y != nullptr);
+    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp, raw_y != nullptr);
     __syncthreads();
     // Phase C: column sums of the raw rows of each relative lightcurve, in point order
     for (int it = tid; it < P.nlc * (ma + 1); it += kWG) {
@@ -364,10 +418,11 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
     __syncthreads();
     // Phase D: alpha/beta/chi2 from the normalised rows (mrqcof.c:89-145)
     const int nb = (mfit + 16) >> 4;  // 16-column blocks covering mfit + 1 columns
-    d4 acc[10];
+    V4 acc[10];
 #pragma unroll
-    for (int b = 0; b < 10; ++b) acc[b] = d4{0, 0, 0, 0};
-    double *st = sh.u.d.stage[wave];
+    for (int b = 0; b < 10; ++b) acc[b] = Pr::zero();
+    T *st = reinterpret_cast<T *>(sh.u.d.stage[wave]);
+    double chi = 0;  // MIX: this lane's share of sum w*r^2 (FP64)
     const int nsteps = (npts + 3) >> 2;
     for (int s = wave; s < nsteps; s += kWaves) {
         const int p = 4 * s + kg;
@@ -399,23 +454,29 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
                     if (rel) v = (col == 0) ? 0.0 : coef * (v - yt * dv[col] / ave);
                 } else if (c == mfit) {
                     v = P.y[p] - (rel ? coef * yt : yt);
+                    if (MIX) chi += wq * v * v;
                 }
             }
-            st[kg * 64 + c] = v;
+            st[kg * 64 + c] = T(v);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double vb[4];
+        T vb[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) vb[b] = (b < nb) ? st[kg * 64 + 16 * b + pl] : 0.0;
+        for (int b = 0; b < 4; ++b) vb[b] = (b < nb) ? st[kg * 64 + 16 * b + pl] : T(0);
+        const T wt = T(wq);
         int idx = 0;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj <= bi; ++bj, ++idx)
-                if (bi < nb) acc[idx] = mfma(vb[bi], wq * vb[bj], acc[idx]);
+                if (bi < nb) acc[idx] = Pr::mma(vb[bi], wt * vb[bj], acc[idx]);
         __builtin_amdgcn_wave_barrier();
+    }
+    if (MIX) {
+        for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off);
+        if (lane == 0) sh.chi_part[wave] = chi;
     }
     __syncthreads();
     // deterministic cross-wave sum into u.d.red (lower blocks; diagonal blocks in full)
@@ -429,8 +490,9 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
                     if (bi < nb)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int o = (16 * bi + 4 * r + kg) * 64 + 16 * bj + pl;
-                            sh.u.d.red[o] = (w == 0) ? acc[idx][r] : sh.u.d.red[o] + acc[idx][r];
+                            const int o = (16 * bi + Pr::row(r, kg)) * 64 + 16 * bj + pl;
+                            const double v = double(acc[idx][r]);
+                            sh.u.d.red[o] = (w == 0) ? v : sh.u.d.red[o] + v;
                         }
         }
         __syncthreads();
@@ -440,7 +502,13 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
         alpha[it] = (j >= k) ? sh.u.d.red[j * 64 + k] : sh.u.d.red[k * 64 + j];
     }
     for (int j = tid; j < mfit; j += kWG) beta[j] = sh.u.d.red[mfit * 64 + j];
-    const double chisq = sh.u.d.red[mfit * 64 + mfit];
+    double chisq;
+    if (MIX) {
+        chisq = 0;
+        for (int w = 0; w < kWaves; ++w) chisq += sh.chi_part[w];
+    } else {
+        chisq = sh.u.d.red[mfit * 64 + mfit];
+    }
     if (raw_y) {
         for (int p = tid; p < npts; p += kWG) raw_y[p] = ytemp[p];
         for (int it = tid; it < npts * ma; it += kWG) {
@@ -460,7 +528,7 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
 // ---------------------------------------------------------------------------------------------
 __device__ inline int gauss_jordan(Shared &sh, int n) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *a = sh.u.cov, *b = sh.da;
+    double *a = sh.u.g.cov, *b = sh.da;
     for (int j = tid; j < n; j += kWG) sh.ipiv[j] = 0;
     __syncthreads();
     for (int i = 0; i < n; ++i) {
@@ -503,6 +571,10 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         __syncthreads();
         const int irow = sh.iscal[0], icol = sh.iscal[1];
         if (irow < 0) return 1;
+        if (tid == 0) {
+            sh.indxr[i] = irow;
+            sh.indxc[i] = icol;
+        }
         if (irow != icol)
             for (int l = tid; l <= n; l += kWG) {
                 double *x = (l < n) ? &a[irow * n + l] : &b[irow];
@@ -539,18 +611,51 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
     return 0;
 }
 
+// One step of iterative refinement of the Gauss-Jordan solution x (in sh.da) of A x = b, with A and b
+// kept in sh.u.g.cov0 / sh.b0: unscramble the inverse's columns (gauss_errc.c:63-67), r = b - A x in
+// FP64, x += A^-1 r.
+__device__ inline void refine_solution(Shared &sh, int n) {
+    const int tid = threadIdx.x;
+    double *inv = sh.u.g.cov, *A = sh.u.g.cov0;
+    for (int l = n - 1; l >= 0; --l) {
+        const int r0 = sh.indxr[l], c0 = sh.indxc[l];
+        if (r0 != c0)
+            for (int k = tid; k < n; k += kWG) {
+                const double t = inv[k * n + r0];
+                inv[k * n + r0] = inv[k * n + c0];
+                inv[k * n + c0] = t;
+            }
+        __syncthreads();
+    }
+    for (int j = tid; j < n; j += kWG) {
+        double s = sh.b0[j];
+        for (int k = 0; k < n; ++k) s -= A[j * n + k] * sh.da[k];
+        sh.col[j] = s;
+    }
+    __syncthreads();
+    double dx = 0;
+    if (tid < n)
+        for (int k = 0; k < n; ++k) dx += inv[tid * n + k] * sh.col[k];
+    __syncthreads();
+    if (tid < n) sh.da[tid] += dx;
+    __syncthreads();
+}
+
 __device__ inline void load_facets(const DevProblem &P, Shared &sh) {
     for (int i = threadIdx.x; i < P.nf; i += kWG) {
         sh.nx[i] = P.nx[i];
         sh.ny[i] = P.ny[i];
         sh.nz[i] = P.nz[i];
+        sh.nxf[i] = float(P.nx[i]);
+        sh.nyf[i] = float(P.ny[i]);
+        sh.nzf[i] = float(P.nz[i]);
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // One (period, pole) fit per workgroup iteration: period_search_BOINC.cpp:819-911 with mrqmin.c.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const double *freq, const int *pole, int nfits,
                                                   double conw_r, ps_fit_result *out) {
     __shared__ Shared sh;
@@ -595,19 +700,25 @@ __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const d
                 alamda = P.alamda_start;
                 if (tid < ma) sh.a[tid] = sh.cg[tid];
                 __syncthreads();
-                ochisq = mrqcof<NT, R, LDSB>(P, J, ytemp, dave, cur ? alpha1 : alpha0, cur ? beta1 : beta0, sh, nullptr,
+                ochisq = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, cur ? alpha1 : alpha0, cur ? beta1 : beta0, sh, nullptr,
                                              nullptr);
                 if (tid < ma) sh.atry[tid] = sh.cg[tid];
             }
             for (int it = tid; it < mfit * mfit; it += kWG) {
                 const int j = it / mfit, k = it - j * mfit;
                 const double v = (cur ? alpha1 : alpha0)[it];
-                sh.u.cov[it] = (j == k) ? v * (1 + alamda) : v;
+                const double d = (j == k) ? v * (1 + alamda) : v;
+                sh.u.g.cov[it] = d;
+                if (MIX) sh.u.g.cov0[it] = d;
             }
-            for (int j = tid; j < mfit; j += kWG) sh.da[j] = (cur ? beta1 : beta0)[j];
+            for (int j = tid; j < mfit; j += kWG) {
+                sh.da[j] = (cur ? beta1 : beta0)[j];
+                if (MIX) sh.b0[j] = sh.da[j];
+            }
             __syncthreads();
             const int err = gauss_jordan(sh, mfit);
             __syncthreads();
+            if (MIX && err == 0) refine_solution(sh, mfit);
             if (err == 0) {
                 if (tid < mfit) {
                     const int l = P.fit_idx[tid];
@@ -616,7 +727,7 @@ __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const d
                 __syncthreads();
                 if (tid < ma) sh.a[tid] = sh.atry[tid];
                 __syncthreads();
-                chisq = mrqcof<NT, R, LDSB>(P, J, ytemp, dave, cur ? alpha0 : alpha1, cur ? beta0 : beta1, sh, nullptr,
+                chisq = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, cur ? alpha0 : alpha1, cur ? beta0 : beta1, sh, nullptr,
                                             nullptr);
                 if (chisq < ochisq) {
                     alamda = alamda / P.alamda_incr;
@@ -676,7 +787,7 @@ __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const d
 }
 
 // mrqcof for externally supplied parameter vectors (ps_eval).
-template <int NT, int R, bool LDSB>
+template <int NT, int R, bool LDSB, bool MIX>
 __global__ __launch_bounds__(kWG) void eval_kernel(DevProblem P, Slots S, const double *avec, int nvec, double *alpha,
                                                    double *beta, double *chisq, double *raw_y, double *raw_dyda) {
     __shared__ Shared sh;
@@ -691,7 +802,7 @@ __global__ __launch_bounds__(kWG) void eval_kernel(DevProblem P, Slots S, const 
     for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
         if (tid < P.ma) sh.a[tid] = avec[size_t(v) * P.ma + tid];
         __syncthreads();
-        const double c = mrqcof<NT, R, LDSB>(P, J, ytemp, dave, al, be, sh,
+        const double c = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, al, be, sh,
                                              raw_y ? raw_y + size_t(v) * P.npts : nullptr,
                                              raw_dyda ? raw_dyda + size_t(v) * P.npts * P.ma : nullptr);
         for (int it = tid; it < mfit * mfit; it += kWG)
