@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "trial-period LM fits/sec at Lmax=6, 288 facets: mixed vs FP64, same best period"
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
 ORACLE_CLI = os.path.join(ROOT, "oracle", "ps_oracle_cli")
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "ps_ref_cli")
 
 
 def algorithmic_flops_per_eval(info):
@@ -42,18 +43,25 @@ def traffic_per_launch(workload, precision):
     return t.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(input_path, conw_r, info, seconds_target=20.0):
-    """The oracle (FP64 CPU restatement, one fit per process at a time) on the host's cores: each
-    process runs all 10 poles of one trial period, or fewer poles to stay near seconds_target."""
-    if not os.path.exists(ORACLE_CLI):
+def cpu_baseline(input_path, conw_r, seconds_target=20.0):
+    """The reference's own plain-C core (oracle/_ref/ps_ref_cli, compiled from the reference sources by
+    oracle/Makefile; kind "reference") or, where that was not built, the oracle's FP64 restatement
+    (oracle/ps_oracle_cli; kind "port"), single-threaded as the reference is, one process per host core
+    (at most 16) on disjoint trial periods of the same workload -- the reference's own data-parallel
+    split.  Each process runs whole periods (10 poles) for about seconds_target."""
+    if os.path.exists(REF_CLI):
+        cli, kind = REF_CLI, "reference"
+    elif os.path.exists(ORACLE_CLI):
+        cli, kind = ORACLE_CLI, "port"
+    else:
         return None
-    cores = max(1, min(16, os.cpu_count() or 1))
-    probe = subprocess.run([ORACLE_CLI, "fits", input_path, "1", "1", repr(conw_r), "1"], capture_output=True,
-                           text=True, check=True)
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    probe = subprocess.run([cli, "fits", input_path, "1", "1", repr(conw_r), "1"], capture_output=True, text=True,
+                           check=True)
     t_fit = float(probe.stdout.split()[3])
-    npoles = max(1, min(10, int(seconds_target / max(t_fit, 1e-3))))
+    nper = max(1, int(round(seconds_target / (10 * max(t_fit, 1e-3)))))
     t0 = time.perf_counter()
-    procs = [subprocess.Popen([ORACLE_CLI, "fits", input_path, str(2 + k), str(2 + k), repr(conw_r), str(npoles)],
+    procs = [subprocess.Popen([cli, "fits", input_path, str(2 + k * nper), str(1 + (k + 1) * nper), repr(conw_r)],
                               stdout=subprocess.PIPE, text=True) for k in range(cores)]
     nfits = 0
     for p in procs:
@@ -62,9 +70,23 @@ def cpu_baseline(input_path, conw_r, info, seconds_target=20.0):
             return None
         nfits += int(out.split()[1])
     wall = time.perf_counter() - t0
-    return {"value": nfits / wall, "unit": "fits/s", "cores": cores, "kind": "port",
-            "sample": f"{cores} processes x 1 trial period x {npoles} poles of the same workload "
-                      f"({nfits} fits, {wall:.1f} s wall), oracle/ps_oracle_cli FP64 restatement"}
+    return {"value": round(nfits / wall, 3), "unit": "fits/s", "cores": cores, "kind": kind,
+            "sample": f"{cores} processes x {nper} trial period(s) x 10 poles of the same workload "
+                      f"({nfits} fits, {wall:.1f} s wall), {os.path.relpath(cli, ROOT)}"}
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counter readings turn
+    the fit kernel's readings into bytes."""
+    import torch
+    n = (mib << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8, "fetch_kib_expected": n * 8 / 1024, "write_kib_expected": n * 8 / 1024}),
+          flush=True)
 
 
 def main():
@@ -77,7 +99,12 @@ def main():
                     help="also run the other precision on the same fits: speedup and best-period agreement")
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one plain device copy of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale factors)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the scan shards by "
               "independent period ranges, see DESIGN.md", file=sys.stderr)
@@ -141,7 +168,7 @@ def main():
                      "frac": round(achieved / peak, 4),
                      "traffic": traffic_per_launch(workload, args.precision),
                      "flops_per_fit": f_fit, "avg_launch_ms": round(avg_launch_s * 1e3, 3)},
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(input_path, conw_r, info),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(input_path, conw_r),
     }
     ps.close()
     print(json.dumps(line), flush=True)

@@ -129,7 +129,8 @@ void ps_geometry_free(ps_geometry *geo);
 
 /* ---- device ---- */
 /* Uploads points, lightcurve table, facet tables and the free/fixed mask; allocates per-slot scratch.
-   Returns PS_ERR_UNSUPPORTED when mfit > 64 or nfac > 1024 (lmax <= 6 is supported). */
+   Returns PS_ERR_UNSUPPORTED when mfit > 63, ma > 72 or nfac > 512 (Lmax <= 6 and nrows <= 8 are
+   supported; the reference allows up to MAX_N_PAR = 200 parameters and MAX_N_FAC = 1000 facets). */
 int ps_prepare(const ps_input *in, const ps_geometry *geo, int device, ps_handle **out);
 int ps_set_precision(ps_handle *h, int precision);
 int ps_get_info(const ps_handle *h, ps_info *info);

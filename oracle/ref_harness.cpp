@@ -70,6 +70,7 @@ int psr_gauss_errc(double *a, int n, double *b);
 int psr_fit(double freq, int m, double *out7);
 int psr_run(int n_from, int n_to, double *lines, double *conw_out);
 int psr_load_file(const char *path);
+int psr_freq(double *f0, double *df, int *nmax);
 }
 
 static struct {
@@ -386,4 +387,12 @@ int psr_load_file(const char *path) {
     if (rc == 0) rc = psr_setup(&p);
     pso_problem_free(&p);
     return rc;
+}
+
+/* Frequency grid of the loaded problem (period_search_BOINC.cpp:784-806). */
+int psr_freq(double *f0, double *df, int *nmax) {
+    if (f0) *f0 = H.freq_start;
+    if (df) *df = H.freq_step;
+    if (nmax) *nmax = (int)((H.freq_start - H.freq_end) / H.freq_step) + 1;
+    return 0;
 }
