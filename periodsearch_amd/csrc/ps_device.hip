@@ -65,6 +65,7 @@ struct ps_handle {
     std::vector<double> s2i_host;  // 1/sig^2 per point (convexity rows set per call)
     double conw_loaded = -1;
     DevProblem P{};
+    const DevProblem *dP = nullptr;  // device copy of P: the kernels read its fields on demand
     Slots S{};
     Variant var{};
     double *d_s2i = nullptr;
@@ -89,14 +90,14 @@ namespace {
 
 template <int NT, int R, bool LDSB, bool MIX>
 int launch_fit(ps_handle *h, int grid, int nfits, double conw_r) {
-    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, h->d_freq,
+    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->dP, h->S, h->d_freq,
                        h->d_pole, nfits, conw_r, h->d_out);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
 template <int NT, int R, bool LDSB, bool MIX>
 int launch_eval(ps_handle *h, int grid, const double *a, int nvec, double *al, double *be, double *ch, double *ry,
                 double *rd) {
-    hipLaunchKernelGGL((psk::eval_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->P, h->S, a, nvec,
+    hipLaunchKernelGGL((psk::eval_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->dP, h->S, a, nvec,
                        al, be, ch, ry, rd);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
@@ -148,9 +149,11 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     for (int l = 0; l < ma; ++l)
         if (ia[l]) fit_idx.push_back(l);
     const int mfit = int(fit_idx.size());
-    if (mfit > kMaxFit || ma > kMaxMa)
-        return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support up to %d fitted parameters (got %d; Lmax <= 6)",
-                       kMaxFit, mfit);
+    // J rows hold ma raw columns plus the residual in column kRes, one lane per column in phase C
+    if (mfit > kMaxFit || ma > psk::kRes)
+        return ps_fail(PS_ERR_UNSUPPORTED,
+                       "GPU kernels support up to %d fitted of %d parameters (got %d of %d; Lmax <= 6)", kMaxFit,
+                       psk::kRes, mfit, ma);
     if (geo->nfac > kMaxFac) return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support nrows <= 8 (nfac <= %d)", kMaxFac);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ps_fail(PS_ERR_DEVICE, "no HIP device");
@@ -295,6 +298,8 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.pole_lambda = dev_upload(pl, PS_N_POLES, o);
     for (void *ptr : o)
         if (!ptr) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
+    h->dP = dev_upload(&P, 1, o);
+    if (!h->dP) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
 
     // resident workgroups: one scratch slot each
     hipDeviceProp_t prop;

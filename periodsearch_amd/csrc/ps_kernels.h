@@ -34,8 +34,10 @@ constexpr int kWaves = kWG / 64;
 constexpr int kMaxFit = 63;     // fitted parameters; +1 residual column = 64
 constexpr int kMaxMa = 72;
 constexpr int kMap = 64;        // J row pitch (doubles)
+constexpr int kRes = kMap - 1;  // J column holding the residual after phase C (ma <= 63)
 constexpr int kMaxFac = 512;    // nrows <= 8
-constexpr int kDsphLds = 320 * 49;  // FP64 Dsph held in LDS (facets x pitch), else read from HBM/L2
+constexpr int kTP = 2;          // 16-point tiles per wave per pass of the facet loop
+constexpr int kDsphLds = 288 * 49;  // FP64 Dsph held in LDS (facets x pitch: nrows <= 6, Lmax <= 6), else read from L2
 constexpr double kTiny = 1e-8;  // period_search/constants.h:10
 constexpr double kPi = M_PI;
 
@@ -70,12 +72,13 @@ struct Shared {
     double a[kMaxMa];     // parameters under evaluation
     double cg[kMaxMa];    // accepted parameters
     double atry[kMaxMa];
-    double nx[kMaxFac], ny[kMaxFac], nz[kMaxFac], area[kMaxFac], g[kMaxFac];
-    float nxf[kMaxFac], nyf[kMaxFac], nzf[kMaxFac], areaf[kMaxFac];  // FP32 copies (mixed precision)
+    double nx[kMaxFac], ny[kMaxFac], nz[kMaxFac];
+    double area[kMaxFac], acl[kMaxFac], acls[kMaxFac];  // Area, Area*cl, Area*cls of the evaluation
+    float nxf[kMaxFac], nyf[kMaxFac], nzf[kMaxFac];     // FP32 copies (mixed precision)
+    float areaf[kMaxFac], aclf[kMaxFac], aclsf[kMaxFac];
     double da[64], col[64], b0[64];
     int indxr[64], indxc[64];
-    double chi_part[kWaves];
-    double red_v[kWaves];
+    double red_v[kWaves], red_s[kWaves];
     int red_q[kWaves];
     int ipiv[64];
     double scal[8];
@@ -89,9 +92,8 @@ struct Shared {
             double cov0[64 * 64];    // the damped matrix kept for iterative refinement
         } g;
         struct {
-            double stage[kWaves][4 * 64];
-            double red[64 * 64];
-        } d;                         // phase D
+            double red[64 * 64];     // phase D: cross-wave sum of the normal equations
+        } d;
     } u;
 };
 
@@ -125,9 +127,11 @@ struct Prec<true> {
 };
 
 // ---------------------------------------------------------------------------------------------
-// curv (period_search/curv.c:10-39): g = exp(sum_{m,l} Pleg*(c*Fc + s*Fs)), Area = Darea*g.
+// curv (period_search/curv.c:10-39): g = exp(sum_{m,l} Pleg*(c*Fc + s*Fs)), Area = Darea*g, and the
+// facet loop's Area*cl and Area*cls.
 // ---------------------------------------------------------------------------------------------
 __device__ inline void curv_lds(const DevProblem &P, const double *a, Shared &sh) {
+    const double cl = exp(a[P.nc + 6]), cls = a[P.nc + 7];
     for (int i = threadIdx.x; i < P.nf; i += kWG) {
         double gsum = 0;
         int n = 0;
@@ -143,9 +147,10 @@ __device__ inline void curv_lds(const DevProblem &P, const double *a, Shared &sh
             }
             gsum = gsum + pl[q] * fsum;
         }
-        const double gv = exp(gsum);
-        sh.g[i] = gv;
-        sh.area[i] = P.darea[i] * gv;
+        const double ar = P.darea[i] * exp(gsum);
+        sh.area[i] = ar;
+        sh.acl[i] = ar * cl;
+        sh.acls[i] = ar * cls;
     }
 }
 
@@ -206,33 +211,39 @@ __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, c
     }
 }
 
-// Derivatives of the body-frame directions w.r.t. beta, lambda, omega (de[i][k]), the phase
-// function and its derivatives (bright.c:30-57, matrix.c:11-58, phasec.c:11-24).
-struct PointDer {
-    double de[3][3], de0[3][3], scale, dphp[3];
-};
+// Phase function of point p and its derivatives w.r.t. the three phase parameters
+// (phasec.c:11-24, bright.c:30-35).
+__device__ inline double phase_scale(const DevProblem &P, const double *a, int p, double dphp[3]) {
+    const int nc = P.nc;
+    const double cos_alpha = P.ex[p] * P.sx[p] + P.ey[p] * P.sy[p] + P.ez[p] * P.sz[p];
+    const double alpha = acos(cos_alpha);
+    const double p1 = a[nc + 3], p2 = a[nc + 4], p3 = a[nc + 5];
+    const double ephi = exp(-alpha / p2);
+    dphp[0] = ephi;
+    dphp[1] = p1 * ephi * alpha / (p2 * p2);
+    dphp[2] = alpha;
+    return 1 + p1 * ephi + p3 * alpha;
+}
 
-__device__ inline void point_derivs(const DevProblem &P, const double *a, int p, const Shared &sh, PointDer &G) {
+// Brightness derivatives w.r.t. beta, lambda, omega before the phase factor (bright.c:88-104):
+// js[j] = sum_i S[i]*de[i][j] + U[i]*de0[i][j], where de / de0 are the derivatives of the body-frame
+// Earth / Sun directions (matrix.c:11-58, blmatrix.c) and S, U the facet sums of dbr/de, dbr/de0.
+// Each de[i][j] is formed and consumed in turn, never held as a table.
+__device__ inline void spin_columns(const DevProblem &P, const double *a, int p, const Shared &sh, const double S[3],
+                                    const double U[3], double js[3]) {
     const int nc = P.nc;
     const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
     const double ee0[3] = {P.sx[p], P.sy[p], P.sz[p]};
     const double t = P.t[p];
-    const double cos_alpha = ee[0] * ee0[0] + ee[1] * ee0[1] + ee[2] * ee0[2];
-    const double alpha = acos(cos_alpha);
-    const double p1 = a[nc + 3], p2 = a[nc + 4], p3 = a[nc + 5];
-    const double ephi = exp(-alpha / p2);
-    G.scale = 1 + p1 * ephi + p3 * alpha;
-    G.dphp[0] = ephi;
-    G.dphp[1] = p1 * ephi * alpha / (p2 * p2);
-    G.dphp[2] = alpha;
     double sf, cf;
     rotation_phase(a[nc + 2], t, P.phi0, sf, cf);
     const double fm[3][3] = {{cf, sf, 0}, {-sf, cf, 0}, {0, 0, 1}};
     const double dfm[3][3] = {{-t * sf, t * cf, 0}, {-t * cf, -t * sf, 0}, {0, 0, 0}};
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+        double acc = 0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {  // de[i][j] = sum_k dtm[j][i][k] ee[k]
+        for (int i = 0; i < 3; ++i) {  // de[i][j] = sum_k dtm[j][i][k] ee[k]
             double d = 0, d0 = 0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -243,9 +254,10 @@ __device__ inline void point_derivs(const DevProblem &P, const double *a, int p,
                 d = d + dt * ee[k];
                 d0 = d0 + dt * ee0[k];
             }
-            G.de[i][j] = d;
-            G.de0[i][j] = d0;
+            acc = acc + (S[i] * d + U[i] * d0);
         }
+        js[j] = acc;
+    }
 }
 
 template <class T>
@@ -258,105 +270,156 @@ __device__ inline T sum_groups(T v) {  // sum over the four 16-lane groups of a 
 // ---------------------------------------------------------------------------------------------
 // Phase B: bright (period_search/bright.c:14-126) for every observed point, conv (conv.c:12-29) for
 // the three convexity rows.  NT full 16-column MFMA tiles + R leftover coefficients (NT*16 + R >= nc).
-// MIX: facet terms, rotation sums and the shape contraction in FP32; the body-frame directions and
-// the brightness (the model value entering the residual) stay FP64.
+// Each wave takes two 16-point tiles per pass: both share every facet and Dsph load and give the
+// lane two independent dependency chains.  Visibility is a 0/1 factor, not a branch: the reciprocal
+// is taken of max(mu + mu0, 1e-8) -- a no-op for visible facets (mu, mu0 > 1e-8) and, for the rest,
+// small enough that (mu0/(mu+mu0))^2 <= 1e16 stays finite in FP32 -- so invisible facets add exact
+// zeros.  With A = Area, Ac = Area*cl, As = Area*cls (per facet, in LDS):
+//   X   = A*s            = Ac*mm + As*mm/(mu+mu0),       mm = vis*mu*mu0
+//   ad  = A*ds/dmu       = vis*(As*(mu0/(mu+mu0))^2 + Ac*mu0),  ad0 likewise with mu
+// (bright.c:69-76).  MIX: facet terms, rotation sums and the shape contraction in FP32; the
+// body-frame directions and the brightness accumulation stay FP64.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB, bool MIX>
-__device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp, bool all_derivs) {
+template <int NT, int R, bool MIX>
+struct TileSums {
     using Pr = Prec<MIX>;
     using T = typename Pr::T;
-    using V4 = typename Pr::V4;
+    typename Pr::V4 acc[NT];
+    T racc[R > 0 ? R : 1];
+    T S[3], U[3], d1, d2;
+    double br;
+};
+
+template <int NT, int R, bool LDSB, bool MIX>
+__device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp) {
+    using Pr = Prec<MIX>;
+    using T = typename Pr::T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int pl = lane & 15, kg = lane >> 4;
     const int nc = P.nc, pitch = P.dpitch;
     const double *a = sh.a;
-    const T *dsph, *snx, *sny, *snz, *sar;
+    const T *dsph, *snx, *sny, *snz, *sar, *sac, *sas;
     if constexpr (MIX) {
         dsph = LDSB ? sh.u.dsphf : P.dsphf;
-        snx = sh.nxf; sny = sh.nyf; snz = sh.nzf; sar = sh.areaf;
+        snx = sh.nxf; sny = sh.nyf; snz = sh.nzf; sar = sh.areaf; sac = sh.aclf; sas = sh.aclsf;
     } else {
         dsph = LDSB ? sh.u.dsph : P.dsph;
-        snx = sh.nx; sny = sh.ny; snz = sh.nz; sar = sh.area;
+        snx = sh.nx; sny = sh.ny; snz = sh.nz; sar = sh.area; sac = sh.acl; sas = sh.acls;
     }
     const double cl_d = exp(a[nc + 6]);
-    const T cl = T(cl_d), cls = T(a[nc + 7]);
-    const bool want_d1 = P.ia_cl || all_derivs, want_d2 = P.ia_ls || all_derivs;
     const int ntiles = (P.nobs + 15) >> 4;
-    const int nchunk = P.nf >> 2;
-    for (int tile = wave; tile < ntiles; tile += kWaves) {
-        const int p = tile * 16 + pl;
-        const bool valid = p < P.nobs;
-        double ed[3], e0d[3];
-        point_dirs(P, a, valid ? p : 0, sh, ed, e0d);
-        const T e[3] = {T(ed[0]), T(ed[1]), T(ed[2])}, e0[3] = {T(e0d[0]), T(e0d[1]), T(e0d[2])};
-        V4 acc[NT];
+    const int nchunk = P.nf >> 2;  // nf = 8 nrows^2
+    for (int tp = wave; kTP * tp < ntiles; tp += kWaves) {
+        T e[kTP][3], e0[kTP][3];
+        bool valid[kTP];
+        TileSums<NT, R, MIX> ts[kTP];
 #pragma unroll
-        for (int tt = 0; tt < NT; ++tt) acc[tt] = Pr::zero();
-        T racc[R > 0 ? R : 1];
+        for (int h = 0; h < kTP; ++h) {
+            const int p = (kTP * tp + h) * 16 + pl;
+            valid[h] = p < P.nobs;
+            double ed[3], e0d[3];
+            point_dirs(P, a, valid[h] ? p : 0, sh, ed, e0d);
 #pragma unroll
-        for (int r = 0; r < R; ++r) racc[r] = 0;
-        double br = 0;
-        T S0 = 0, S1 = 0, S2 = 0, T0 = 0, T1 = 0, T2 = 0, d1 = 0, d2 = 0;
+            for (int k = 0; k < 3; ++k) {
+                e[h][k] = T(ed[k]);
+                e0[h][k] = T(e0d[k]);
+                ts[h].S[k] = 0;
+                ts[h].U[k] = 0;
+            }
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) ts[h].acc[tt] = Pr::zero();
+#pragma unroll
+            for (int r = 0; r < R; ++r) ts[h].racc[r] = 0;
+            ts[h].d1 = ts[h].d2 = 0;
+            ts[h].br = 0;
+        }
         for (int c = 0; c < nchunk; ++c) {
             const int j = 4 * c + kg;
-            const T n0 = snx[j], n1 = sny[j], n2 = snz[j], A = sar[j];
-            const T mu = e[0] * n0 + e[1] * n1 + e[2] * n2;
-            const T mu0 = e0[0] * n0 + e0[1] * n1 + e0[2] * n2;
-            const bool vis = valid && (mu > T(kTiny)) && (mu0 > T(kTiny));
-            const T rd = Pr::rcp(mu + mu0);
-            const T s = mu * mu0 * (cl + cls * rd);
-            const T X = vis ? A * s : T(0);
-            br += double(X);
-            const T q0 = mu0 * rd, q = mu * rd;
-            const T ad = vis ? A * (cls * (q0 * q0) + cl * mu0) : T(0);
-            const T ad0 = vis ? A * (cls * (q * q) + cl * mu) : T(0);
-            S0 += ad * n0; S1 += ad * n1; S2 += ad * n2;
-            T0 += ad0 * n0; T1 += ad0 * n1; T2 += ad0 * n2;
-            if (want_d1) d1 += vis ? mu * mu0 * A : T(0);
-            if (want_d2) d2 += vis ? A * mu * mu0 * rd : T(0);
+            const T n0 = snx[j], n1 = sny[j], n2 = snz[j], A = sar[j], Ac = sac[j], As = sas[j];
             const T *brow = dsph + j * pitch;
+            T B[NT], Br[R > 0 ? R : 1];
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) acc[tt] = Pr::mma(X, brow[16 * tt + pl], acc[tt]);
+            for (int tt = 0; tt < NT; ++tt) B[tt] = brow[16 * tt + pl];
 #pragma unroll
-            for (int r = 0; r < R; ++r) racc[r] += X * brow[16 * NT + r];
+            for (int r = 0; r < R; ++r) Br[r] = brow[16 * NT + r];
+#pragma unroll
+            for (int h = 0; h < kTP; ++h) {
+                TileSums<NT, R, MIX> &s = ts[h];
+                const T mu = e[h][0] * n0 + e[h][1] * n1 + e[h][2] * n2;
+                const T mu0 = e0[h][0] * n0 + e0[h][1] * n1 + e0[h][2] * n2;
+                const T vis = (valid[h] && mu > T(kTiny) && mu0 > T(kTiny)) ? T(1) : T(0);
+                const T rd = Pr::rcp(fmax(mu + mu0, T(kTiny)));
+                const T mm = vis * (mu * mu0);
+                const T mr = mm * rd;
+                const T X = Ac * mm + As * mr;
+                s.br += double(X);
+                s.d1 += A * mm;
+                s.d2 += A * mr;
+                const T q0 = mu0 * rd, q = mu * rd;
+                const T ad = vis * (As * (q0 * q0) + Ac * mu0);
+                const T ad0 = vis * (As * (q * q) + Ac * mu);
+                s.S[0] += ad * n0; s.S[1] += ad * n1; s.S[2] += ad * n2;
+                s.U[0] += ad0 * n0; s.U[1] += ad0 * n1; s.U[2] += ad0 * n2;
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) s.acc[tt] = Pr::mma(X, B[tt], s.acc[tt]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) s.racc[r] += X * Br[r];
+            }
         }
-        br = sum_groups(br);
-        S0 = sum_groups(S0); S1 = sum_groups(S1); S2 = sum_groups(S2);
-        T0 = sum_groups(T0); T1 = sum_groups(T1); T2 = sum_groups(T2);
-        if (want_d1) d1 = sum_groups(d1);
-        if (want_d2) d2 = sum_groups(d2);
+        // Epilogue.  The phase factors of both tiles first, so that the MFMA accumulators (the shape
+        // columns: register r of lane l holds point Pr::row(r, kg), coefficient 16tt + pl) are written
+        // and released before the spin columns are formed.
+        double scale[kTP], dphp[kTP][3];
 #pragma unroll
-        for (int r = 0; r < R; ++r) racc[r] = sum_groups(racc[r]);
-        PointDer G;
-        point_derivs(P, a, valid ? p : 0, sh, G);
-        const double scale = G.scale;
-        if (kg == 0 && valid) {
-            double *jr = J + size_t(p) * kMap;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (16 * NT + r < nc) jr[16 * NT + r] = scale * double(racc[r]);
-            const double s0 = S0, s1 = S1, s2 = S2, t0 = T0, t1 = T1, t2 = T2;
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                jr[nc + k] = scale * (s0 * G.de[0][k] + s1 * G.de[1][k] + s2 * G.de[2][k] +
-                                      (t0 * G.de0[0][k] + t1 * G.de0[1][k] + t2 * G.de0[2][k]));
-#pragma unroll
-            for (int i = 0; i < 3; ++i) jr[nc + 3 + i] = br * G.dphp[i];
-            jr[nc + 6] = scale * double(d1) * cl_d;
-            jr[nc + 7] = scale * double(d2);
-            ytemp[p] = br * scale;
+        for (int h = 0; h < kTP; ++h) {
+            const int p = (kTP * tp + h) * 16 + pl;
+            scale[h] = phase_scale(P, a, valid[h] ? p : 0, dphp[h]);
         }
-        // shape columns: register r of lane l holds point Pr::row(r, kg), coefficient 16tt + pl
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = Pr::row(r, kg);
-            const double sc = __shfl(scale, i);
-            const int pi = tile * 16 + i;
-            if (pi < P.nobs) {
-                double *jr = J + size_t(pi) * kMap;
+        for (int h = 0; h < kTP; ++h) {
+            const int tile = kTP * tp + h;
 #pragma unroll
-                for (int tt = 0; tt < NT; ++tt)
-                    if (16 * tt + pl < nc) jr[16 * tt + pl] = sc * double(acc[tt][r]);
+            for (int r = 0; r < 4; ++r) {
+                const int i = Pr::row(r, kg);
+                const double sc = __shfl(scale[h], i);
+                const int pi = tile * 16 + i;
+                if (pi < P.nobs) {
+                    double *jr = J + size_t(pi) * kMap;
+#pragma unroll
+                    for (int tt = 0; tt < NT; ++tt)
+                        if (16 * tt + pl < nc) jr[16 * tt + pl] = sc * double(ts[h].acc[tt][r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < kTP; ++h) {
+            TileSums<NT, R, MIX> &s = ts[h];
+            const int p = (kTP * tp + h) * 16 + pl;
+            const double br = sum_groups(s.br);
+            double S[3], U[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                S[k] = double(sum_groups(s.S[k]));
+                U[k] = double(sum_groups(s.U[k]));
+            }
+            const double d1 = double(sum_groups(s.d1)), d2 = double(sum_groups(s.d2));
+            T racc[R > 0 ? R : 1];
+#pragma unroll
+            for (int r = 0; r < R; ++r) racc[r] = sum_groups(s.racc[r]);
+            if (kg == 0 && valid[h]) {
+                double js[3];
+                spin_columns(P, a, p, sh, S, U, js);
+                double *jr = J + size_t(p) * kMap;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (16 * NT + r < nc) jr[16 * NT + r] = scale[h] * double(racc[r]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) jr[nc + k] = scale[h] * js[k];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) jr[nc + 3 + i] = br * dphp[h][i];
+                jr[nc + 6] = scale[h] * d1 * cl_d;
+                jr[nc + 7] = scale[h] * d2;
+                ytemp[p] = br * scale[h];
             }
         }
     }
@@ -382,15 +445,15 @@ __device__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *y
 // ---------------------------------------------------------------------------------------------
 // mrqcof (period_search/mrqcof.c:16-148) for the parameters in sh.a: writes alpha[mfit][mfit]
 // (full, symmetric) and beta[mfit], returns chi2.  raw_y / raw_dyda receive the raw model rows.
-// MIX: the normal-equation contraction runs on FP32 MFMA with FP32 accumulation; chi2 (the LM
-// acceptance test) is summed in FP64 from the FP64 residuals.
+// The normal equations are accumulated in FP64 in both precisions: in MIX the Jacobian rows carry
+// the rounding of the FP32 facet loop, but J^T W J is summed exactly enough to stay the Gram matrix of
+// those rows.  (Accumulated in FP32 it is not, and on these systems -- condition numbers 1e8-1e10
+// after damping -- the LM steps along the ill-conditioned directions become noise.)
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool LDSB, bool MIX>
-__device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *dave, double *alpha, double *beta,
+__device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *dave, double *alpha, double *beta,
                          Shared &sh, double *raw_y, double *raw_dyda) {
-    using Pr = Prec<MIX>;
-    using T = typename Pr::T;
-    using V4 = typename Pr::V4;
+    using Pd = Prec<false>;  // phase D arithmetic
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int pl = lane & 15, kg = lane >> 4;
     const int ma = P.ma, npts = P.npts, mfit = P.mfit;
@@ -403,80 +466,109 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
             for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = P.dsph[i];
     }
     if constexpr (MIX)
-        for (int i = tid; i < P.nf; i += kWG) sh.areaf[i] = float(sh.area[i]);
+        for (int i = tid; i < P.nf; i += kWG) {
+            sh.areaf[i] = float(sh.area[i]);
+            sh.aclf[i] = float(sh.acl[i]);
+            sh.aclsf[i] = float(sh.acls[i]);
+        }
     __syncthreads();
-    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp, raw_y != nullptr);
+    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp);
     __syncthreads();
-    // Phase C: column sums of the raw rows of each relative lightcurve, in point order
-    for (int it = tid; it < P.nlc * (ma + 1); it += kWG) {
-        const int L = it / (ma + 1), l = it - L * (ma + 1);
-        if (!P.lc_inrel[L]) continue;
-        double s = 0;
-        for (int p = P.lc_start[L]; p < P.lc_start[L + 1]; ++p) s += (l < ma) ? J[size_t(p) * kMap + l] : ytemp[p];
-        dave[it] = s;
+    if (raw_y) {  // raw bright()/conv() rows, before the lightcurve normalisation below
+        for (int p = tid; p < npts; p += kWG) raw_y[p] = ytemp[p];
+        for (int it = tid; it < npts * ma; it += kWG) {
+            const int p = it / ma, k = it - p * ma;
+            raw_dyda[it] = J[size_t(p) * kMap + k];
+        }
+    }
+    // Phase C: relative-lightcurve normalisation in place (mrqcof.c:53-104).  One wave per lightcurve,
+    // lanes on columns (coalesced rows): the column sums dave_l = sum dyda_l and ave = sum ymod over
+    // the curve's points, then each row becomes coef*(dyda_l - ymod*dave_l/ave) with
+    // coef = sig*Lpoints/ave, dyda for c00 = 0, and the residual y - coef*ymod goes to column kRes.
+    // Calibrated curves and the convexity rows keep their rows; their residual is y - ymod.
+    for (int L = wave; L <= P.nlc; L += kWaves) {
+        const int p0 = (L < P.nlc) ? P.lc_start[L] : P.nobs, p1 = (L < P.nlc) ? P.lc_start[L + 1] : npts;
+        if (L < P.nlc && P.lc_inrel[L]) {
+            const int l = lane;  // ma + 1 <= 64 columns (checked by ps_prepare)
+            const double *src = (l < ma) ? J + l : ytemp;
+            const size_t st = (l < ma) ? kMap : 1;
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            if (l <= ma) {
+                int p = p0;
+                for (; p + 4 <= p1; p += 4) {
+                    s0 += src[size_t(p) * st];
+                    s1 += src[size_t(p + 1) * st];
+                    s2 += src[size_t(p + 2) * st];
+                    s3 += src[size_t(p + 3) * st];
+                }
+                for (; p < p1; ++p) s0 += src[size_t(p) * st];
+            }
+            const double dv = (s0 + s1) + (s2 + s3);
+            const double ave = __shfl(dv, ma);
+            const double r = dv / ave;
+            const double cpts = double(p1 - p0) / ave;
+            // rows in groups of four: all loads of a group are issued before its stores
+            for (int pb = p0; pb < p1; pb += 4) {
+                double yt[4], sg[4], jv[4], yo[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int pq = (pb + q < p1) ? pb + q : pb;
+                    yt[q] = ytemp[pq];
+                    sg[q] = P.sig[pq];
+                    jv[q] = (l < ma) ? J[size_t(pq) * kMap + l] : 0.0;
+                    yo[q] = (l == kRes) ? P.y[pq] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (pb + q >= p1) break;
+                    const double coef = sg[q] * cpts;
+                    double *jr = J + size_t(pb + q) * kMap;
+                    if (l < ma) jr[l] = (l == 0) ? 0.0 : coef * (jv[q] - yt[q] * r);
+                    if (l == kRes) jr[kRes] = yo[q] - coef * yt[q];
+                }
+            }
+        } else {
+            for (int p = p0 + lane; p < p1; p += 64) J[size_t(p) * kMap + kRes] = P.y[p] - ytemp[p];
+        }
     }
     __syncthreads();
-    // Phase D: alpha/beta/chi2 from the normalised rows (mrqcof.c:89-145)
+    // Phase D: alpha/beta/chi2 (mrqcof.c:106-145).  Each wave takes 4-point K-steps; lane (pl, kg) holds
+    // row 4s + kg, column 16u + pl of the normalised fitted columns plus the residual as column mfit,
+    // which is at once the MFMA A operand (column x point) and, times w/sig^2, the B operand.
     const int nb = (mfit + 16) >> 4;  // 16-column blocks covering mfit + 1 columns
-    V4 acc[10];
+    int colu[4];
 #pragma unroll
-    for (int b = 0; b < 10; ++b) acc[b] = Pr::zero();
-    T *st = reinterpret_cast<T *>(sh.u.d.stage[wave]);
-    double chi = 0;  // MIX: this lane's share of sum w*r^2 (FP64)
+    for (int u = 0; u < 4; ++u) {
+        const int cc = pl + 16 * u;
+        colu[u] = (cc < mfit) ? P.fit_idx[cc] : (cc == mfit) ? kRes : -1;
+    }
+    Pd::V4 acc[10];
+#pragma unroll
+    for (int b = 0; b < 10; ++b) acc[b] = Pd::zero();
     const int nsteps = (npts + 3) >> 2;
-    for (int s = wave; s < nsteps; s += kWaves) {
+    // row loads of step s: four columns of point 4s + kg and its weight w/sig^2
+    auto load_step = [&](int s, double v[4], double &wq) {
         const int p = 4 * s + kg;
-        double wq = 0, coef = 1, yt = 0, ave = 1;
-        const double *dv = nullptr;
-        bool rel = false;
-        if (p < npts) {
-            wq = P.s2i[p] * P.w[p];
-            yt = ytemp[p];
-            if (p < P.nobs) {
-                const int L = P.lc_of[p];
-                if (P.lc_inrel[L]) {
-                    rel = true;
-                    dv = dave + size_t(L) * (ma + 1);
-                    ave = dv[ma];
-                    coef = P.sig[p] * (P.lc_start[L + 1] - P.lc_start[L]) / ave;
-                }
-            }
-        }
-        const double *jr = J + size_t(p < npts ? p : 0) * kMap;
+        const bool in = p < npts;
+        const double *jr = J + size_t(in ? p : 0) * kMap;
+        wq = in ? P.s2i[p] * P.w[p] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int c = pl + 16 * u;
-            double v = 0;
-            if (p < npts) {
-                if (c < mfit) {
-                    const int col = P.fit_idx[c];
-                    v = jr[col];
-                    if (rel) v = (col == 0) ? 0.0 : coef * (v - yt * dv[col] / ave);
-                } else if (c == mfit) {
-                    v = P.y[p] - (rel ? coef * yt : yt);
-                    if (MIX) chi += wq * v * v;
-                }
-            }
-            st[kg * 64 + c] = T(v);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        T vb[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) vb[b] = (b < nb) ? st[kg * 64 + 16 * b + pl] : T(0);
-        const T wt = T(wq);
+        for (int u = 0; u < 4; ++u) v[u] = (u < nb && colu[u] >= 0) ? jr[colu[u]] : 0.0;
+    };
+    double vcur[4], wcur;
+    if (wave < nsteps) load_step(wave, vcur, wcur);
+    for (int s = wave; s < nsteps; s += kWaves) {
+        double vnext[4], wnext = 0;
+        if (s + kWaves < nsteps) load_step(s + kWaves, vnext, wnext);  // in flight during the MFMAs below
         int idx = 0;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-            for (int bj = 0; bj <= bi; ++bj, ++idx)
-                if (bi < nb) acc[idx] = Pr::mma(vb[bi], wt * vb[bj], acc[idx]);
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (MIX) {
-        for (int off = 32; off > 0; off >>= 1) chi += __shfl_xor(chi, off);
-        if (lane == 0) sh.chi_part[wave] = chi;
+            for (int bj = 0; bj <= bi; ++bj, ++idx)  // blocks beyond nb add zeros
+                acc[idx] = Pd::mma(vcur[bi], wcur * vcur[bj], acc[idx]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vcur[u] = vnext[u];
+        wcur = wnext;
     }
     __syncthreads();
     // deterministic cross-wave sum into u.d.red (lower blocks; diagonal blocks in full)
@@ -490,8 +582,8 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
                     if (bi < nb)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int o = (16 * bi + Pr::row(r, kg)) * 64 + 16 * bj + pl;
-                            const double v = double(acc[idx][r]);
+                            const int o = (16 * bi + Pd::row(r, kg)) * 64 + 16 * bj + pl;
+                            const double v = acc[idx][r];
                             sh.u.d.red[o] = (w == 0) ? v : sh.u.d.red[o] + v;
                         }
         }
@@ -502,109 +594,94 @@ __device__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *
         alpha[it] = (j >= k) ? sh.u.d.red[j * 64 + k] : sh.u.d.red[k * 64 + j];
     }
     for (int j = tid; j < mfit; j += kWG) beta[j] = sh.u.d.red[mfit * 64 + j];
-    double chisq;
-    if (MIX) {
-        chisq = 0;
-        for (int w = 0; w < kWaves; ++w) chisq += sh.chi_part[w];
-    } else {
-        chisq = sh.u.d.red[mfit * 64 + mfit];
-    }
-    if (raw_y) {
-        for (int p = tid; p < npts; p += kWG) raw_y[p] = ytemp[p];
-        for (int it = tid; it < npts * ma; it += kWG) {
-            const int p = it / ma, k = it - p * ma;
-            raw_dyda[it] = J[size_t(p) * kMap + k];
-        }
-    }
+    const double chisq = sh.u.d.red[mfit * 64 + mfit];
     __syncthreads();
     return chisq;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Full-pivot Gauss-Jordan on sh.u.cov (n x n) with right-hand side sh.da (gauss_errc.c:9-73).
-// Pivot: the largest |a| over unpivoted rows/columns, ties to the last one in row-major order
-// (the reference's `>=` scan).  Returns 0, 1 (no valid pivot: all candidates NaN) or 2 (zero pivot).
+// Full-pivot Gauss-Jordan on sh.u.cov (n x n, n <= 63) with right-hand side sh.da (gauss_errc.c:9-73).
+// Pivot: the largest |a| over unpivoted rows and columns, ties to the last one in row-major order
+// (the reference's `>=` scan); NaN entries are never chosen.  Returns 0, 1 (no valid pivot) or
+// 2 (zero pivot).  Lane l of every wave owns column l (l <= n, column n = the right-hand side), wave w
+// rows w, w + 8, ...; the search for the next pivot runs inside the elimination pass over the
+// updated entries, and the row swap and pivot-row scaling share one pass (three barriers per pivot).
 // The inverse's column unscramble is skipped: only the solution is used (mrqmin.c:61-79).
 // ---------------------------------------------------------------------------------------------
+struct PivotCand {
+    double v, s;  // |a|, a
+    int q;        // row * n + column, -1 = none
+};
+
+__device__ inline void pivot_take(PivotCand &c, double s, int q) {
+    const double v = fabs(s);
+    if (v > c.v || (v == c.v && q > c.q)) c = {v, s, q};
+}
+
 __device__ inline int gauss_jordan(Shared &sh, int n) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
     double *a = sh.u.g.cov, *b = sh.da;
-    for (int j = tid; j < n; j += kWG) sh.ipiv[j] = 0;
-    __syncthreads();
+    unsigned long long done = 0;  // pivoted rows / columns
+    PivotCand c{-1.0, 0.0, -1};
+    if (l < n)
+        for (int ll = wave; ll < n; ll += kWaves) pivot_take(c, a[ll * n + l], ll * n + l);
     for (int i = 0; i < n; ++i) {
-        double bv = -1.0;
-        int bq = -1;
-        for (int q = tid; q < n * n; q += kWG) {
-            const int j = q / n, k = q - j * n;
-            if (sh.ipiv[j] != 1 && sh.ipiv[k] == 0) {
-                const double v = fabs(a[q]);
-                if (v >= bv) {
-                    bv = v;
-                    bq = q;
-                }
-            }
-        }
+        // workgroup argmax: waves by shuffles, then every lane combines the eight wave results
         for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int oq = __shfl_xor(bq, off);
-            if (ov > bv || (ov == bv && oq > bq)) {
-                bv = ov;
-                bq = oq;
-            }
+            const PivotCand o{__shfl_xor(c.v, off), __shfl_xor(c.s, off), __shfl_xor(c.q, off)};
+            if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
         }
-        if (lane == 0) {
-            sh.red_v[wave] = bv;
-            sh.red_q[wave] = bq;
+        if (l == 0) {
+            sh.red_v[wave] = c.v;
+            sh.red_s[wave] = c.s;
+            sh.red_q[wave] = c.q;
         }
         __syncthreads();
-        if (tid == 0) {
-            double v = sh.red_v[0];
-            int q = sh.red_q[0];
-            for (int w = 1; w < kWaves; ++w)
-                if (sh.red_v[w] > v || (sh.red_v[w] == v && sh.red_q[w] > q)) {
-                    v = sh.red_v[w];
-                    q = sh.red_q[w];
-                }
-            sh.iscal[0] = q < 0 ? -1 : q / n;
-            sh.iscal[1] = q < 0 ? -1 : q % n;
+        c = {sh.red_v[0], sh.red_s[0], sh.red_q[0]};
+        for (int w = 1; w < kWaves; ++w) {
+            const PivotCand o{sh.red_v[w], sh.red_s[w], sh.red_q[w]};
+            if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
         }
-        __syncthreads();
-        const int irow = sh.iscal[0], icol = sh.iscal[1];
-        if (irow < 0) return 1;
+        if (c.q < 0) return 1;
+        const int irow = c.q / n, icol = c.q - irow * n;
+        if (c.s == 0.0) return 2;
         if (tid == 0) {
             sh.indxr[i] = irow;
             sh.indxc[i] = icol;
         }
-        if (irow != icol)
-            for (int l = tid; l <= n; l += kWG) {
-                double *x = (l < n) ? &a[irow * n + l] : &b[irow];
-                double *y = (l < n) ? &a[icol * n + l] : &b[icol];
-                const double tmp = *x;
-                *x = *y;
-                *y = tmp;
+        const double pivinv = 1.0 / c.s;
+        done |= 1ull << icol;
+        // swap rows irow and icol, scale the new pivot row (gauss_errc.c:38-53); sh.col gets the
+        // pivot column of every other row as it stands after the swap
+        if (wave == 0 && l <= n) {
+            double *xr = (l < n) ? &a[irow * n + l] : &b[irow];
+            double *xc = (l < n) ? &a[icol * n + l] : &b[icol];
+            const double vr = *xr, vc = *xc;
+            *xc = ((l == icol) ? 1.0 : vr) * pivinv;
+            if (irow != icol) {
+                *xr = vc;
+                if (l == icol) sh.col[irow] = vc;
             }
-        __syncthreads();
-        const double piv = a[icol * n + icol];
-        if (piv == 0.0) return 2;
-        const double pivinv = 1.0 / piv;
-        if (tid == 0) sh.ipiv[icol] += 1;
-        for (int l = tid; l <= n; l += kWG) {
-            if (l < n)
-                a[icol * n + l] = (l == icol ? 1.0 : a[icol * n + l]) * pivinv;
-            else
-                b[icol] *= pivinv;
         }
         for (int ll = tid; ll < n; ll += kWG)
-            if (ll != icol) sh.col[ll] = a[ll * n + icol];
+            if (ll != icol && ll != irow) sh.col[ll] = a[ll * n + icol];
         __syncthreads();
-        for (int it = tid; it < n * (n + 1); it += kWG) {
-            const int ll = it / (n + 1), l = it - ll * (n + 1);
-            if (ll == icol) continue;
-            const double dum = sh.col[ll];
-            if (l < n)
-                a[ll * n + l] = (l == icol ? 0.0 : a[ll * n + l]) - a[icol * n + l] * dum;
-            else
-                b[ll] -= b[icol] * dum;
+        // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
+        c = {-1.0, 0.0, -1};
+        if (l <= n) {
+            const double pr = (l < n) ? a[icol * n + l] : b[icol];
+            const bool cand_col = l < n && !((done >> l) & 1);
+            for (int ll = wave; ll < n; ll += kWaves) {
+                if (ll == icol) continue;
+                const double dum = sh.col[ll];
+                if (l < n) {
+                    const double v = ((l == icol) ? 0.0 : a[ll * n + l]) - pr * dum;
+                    a[ll * n + l] = v;
+                    if (cand_col && !((done >> ll) & 1)) pivot_take(c, v, ll * n + l);
+                } else {
+                    b[ll] -= pr * dum;
+                }
+            }
         }
         __syncthreads();
     }
@@ -656,8 +733,11 @@ __device__ inline void load_facets(const DevProblem &P, Shared &sh) {
 // One (period, pole) fit per workgroup iteration: period_search_BOINC.cpp:819-911 with mrqmin.c.
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool LDSB, bool MIX>
-__global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const double *freq, const int *pole, int nfits,
-                                                  double conw_r, ps_fit_result *out) {
+__global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__ Pp, Slots S, const double *freq,
+                                                  const int *pole, int nfits, double conw_r, ps_fit_result *out) {
+    // The problem description is read through a pointer (scalar loads on demand) rather than passed
+    // by value: ~40 kernel-argument pointers held in SGPRs would spill into VGPRs.
+    const DevProblem &P = *Pp;
     __shared__ Shared sh;
     const int tid = threadIdx.x;
     const int nc = P.nc, ma = P.ma, mfit = P.mfit;
@@ -696,39 +776,53 @@ __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const d
         double chck0 = 0, chck1 = 0, chck2 = 0;
         int niter = 0, nfail = 0;
         while (((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min)) {
-            if (alamda < 0) {
+            // One mrqcof per pass (a single inlined copy): the first pass of a fit evaluates the
+            // starting point (mrqmin.c:34-52) and goes straight on to the step; later passes solve the
+            // damped system and evaluate the trial point (mrqmin.c:53-79).
+            const bool init = alamda < 0;
+            int err = 0;
+            if (init) {
                 alamda = P.alamda_start;
                 if (tid < ma) sh.a[tid] = sh.cg[tid];
-                __syncthreads();
-                ochisq = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, cur ? alpha1 : alpha0, cur ? beta1 : beta0, sh, nullptr,
-                                             nullptr);
-                if (tid < ma) sh.atry[tid] = sh.cg[tid];
-            }
-            for (int it = tid; it < mfit * mfit; it += kWG) {
-                const int j = it / mfit, k = it - j * mfit;
-                const double v = (cur ? alpha1 : alpha0)[it];
-                const double d = (j == k) ? v * (1 + alamda) : v;
-                sh.u.g.cov[it] = d;
-                if (MIX) sh.u.g.cov0[it] = d;
-            }
-            for (int j = tid; j < mfit; j += kWG) {
-                sh.da[j] = (cur ? beta1 : beta0)[j];
-                if (MIX) sh.b0[j] = sh.da[j];
-            }
-            __syncthreads();
-            const int err = gauss_jordan(sh, mfit);
-            __syncthreads();
-            if (MIX && err == 0) refine_solution(sh, mfit);
-            if (err == 0) {
-                if (tid < mfit) {
-                    const int l = P.fit_idx[tid];
-                    sh.atry[l] = sh.cg[l] + sh.da[tid];
+            } else {
+                for (int it = tid; it < mfit * mfit; it += kWG) {
+                    const int j = it / mfit, k = it - j * mfit;
+                    const double v = (cur ? alpha1 : alpha0)[it];
+                    const double d = (j == k) ? v * (1 + alamda) : v;
+                    sh.u.g.cov[it] = d;
+                    if (MIX) sh.u.g.cov0[it] = d;
+                }
+                for (int j = tid; j < mfit; j += kWG) {
+                    sh.da[j] = (cur ? beta1 : beta0)[j];
+                    if (MIX) sh.b0[j] = sh.da[j];
                 }
                 __syncthreads();
-                if (tid < ma) sh.a[tid] = sh.atry[tid];
+                err = gauss_jordan(sh, mfit);
                 __syncthreads();
-                chisq = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, cur ? alpha0 : alpha1, cur ? beta0 : beta1, sh, nullptr,
-                                            nullptr);
+                if (MIX && err == 0) refine_solution(sh, mfit);
+                if (err == 0) {
+                    if (tid < mfit) {
+                        const int l = P.fit_idx[tid];
+                        sh.atry[l] = sh.cg[l] + sh.da[tid];
+                    }
+                    __syncthreads();
+                    if (tid < ma) sh.a[tid] = sh.atry[tid];
+                }
+            }
+            if (err == 0) {
+                __syncthreads();
+                // the starting point fills the current buffers, a trial point the other ones
+                const bool other = !init;
+                const double chi = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0,
+                                                            (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr);
+                if (init) {
+                    ochisq = chi;
+                    chisq = chi;  // a failed first step then reports this fit's own starting chi2
+                    if (tid < ma) sh.atry[tid] = sh.cg[tid];
+                    __syncthreads();
+                    continue;
+                }
+                chisq = chi;
                 if (chisq < ochisq) {
                     alamda = alamda / P.alamda_incr;
                     cur ^= 1;
@@ -788,8 +882,10 @@ __global__ __launch_bounds__(kWG) void fit_kernel(DevProblem P, Slots S, const d
 
 // mrqcof for externally supplied parameter vectors (ps_eval).
 template <int NT, int R, bool LDSB, bool MIX>
-__global__ __launch_bounds__(kWG) void eval_kernel(DevProblem P, Slots S, const double *avec, int nvec, double *alpha,
-                                                   double *beta, double *chisq, double *raw_y, double *raw_dyda) {
+__global__ __launch_bounds__(kWG) void eval_kernel(const DevProblem *__restrict__ Pp, Slots S, const double *avec,
+                                                   int nvec, double *alpha, double *beta, double *chisq, double *raw_y,
+                                                   double *raw_dyda) {
+    const DevProblem &P = *Pp;
     __shared__ Shared sh;
     const int tid = threadIdx.x;
     const int mfit = P.mfit;
