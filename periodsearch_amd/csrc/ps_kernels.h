@@ -77,6 +77,7 @@ struct Shared {
     float nxf[kMaxFac], nyf[kMaxFac], nzf[kMaxFac];     // FP32 copies (mixed precision)
     float areaf[kMaxFac], aclf[kMaxFac], aclsf[kMaxFac];
     double da[64], col[64], b0[64];
+    double prow[64], srow[64];  // Gauss-Jordan: the new pivot row, the row moved by a swap
     int indxr[64], indxc[64];
     double red_v[kWaves], red_s[kWaves];
     int red_q[kWaves];
@@ -603,10 +604,12 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
 // Full-pivot Gauss-Jordan on sh.u.cov (n x n, n <= 63) with right-hand side sh.da (gauss_errc.c:9-73).
 // Pivot: the largest |a| over unpivoted rows and columns, ties to the last one in row-major order
 // (the reference's `>=` scan); NaN entries are never chosen.  Returns 0, 1 (no valid pivot) or
-// 2 (zero pivot).  Lane l of every wave owns column l (l <= n, column n = the right-hand side), wave w
-// rows w, w + 8, ...; the search for the next pivot runs inside the elimination pass over the
-// updated entries, and the row swap and pivot-row scaling share one pass (three barriers per pivot).
-// The inverse's column unscramble is skipped: only the solution is used (mrqmin.c:61-79).
+// 2 (zero pivot).  Wave w keeps rows w, w + 8, ... in registers for the whole solve, lane l holding
+// column l (column n = the right-hand side).  Per pivot only the new pivot row and, for a row swap,
+// the row moved into the pivot row's place pass through LDS; the pivot-column entry of a row is read
+// from its lane by a shuffle; the search for the next pivot runs inside the elimination.  Two barriers
+// per pivot.  On success the inverse (columns not unscrambled, gauss_errc.c:63-67) and the solution
+// are written back to sh.u.cov / sh.da.
 // ---------------------------------------------------------------------------------------------
 struct PivotCand {
     double v, s;  // |a|, a
@@ -618,13 +621,21 @@ __device__ inline void pivot_take(PivotCand &c, double s, int q) {
     if (v > c.v || (v == c.v && q > c.q)) c = {v, s, q};
 }
 
+constexpr int kGJRows = (kMaxFit + kWaves - 1) / kWaves;  // rows held per wave
+
 __device__ inline int gauss_jordan(Shared &sh, int n) {
-    const int tid = threadIdx.x, l = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform row ownership
     double *a = sh.u.g.cov, *b = sh.da;
-    unsigned long long done = 0;  // pivoted rows / columns
+    double row[kGJRows];
     PivotCand c{-1.0, 0.0, -1};
-    if (l < n)
-        for (int ll = wave; ll < n; ll += kWaves) pivot_take(c, a[ll * n + l], ll * n + l);
+#pragma unroll
+    for (int k = 0; k < kGJRows; ++k) {
+        const int ll = wave + kWaves * k;
+        row[k] = (ll < n && l <= n) ? ((l < n) ? a[ll * n + l] : b[ll]) : 0.0;
+        if (ll < n && l < n) pivot_take(c, row[k], ll * n + l);
+    }
+    unsigned long long done = 0;  // pivoted rows / columns
     for (int i = 0; i < n; ++i) {
         // workgroup argmax: waves by shuffles, then every lane combines the eight wave results
         for (int off = 32; off > 0; off >>= 1) {
@@ -651,39 +662,53 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         }
         const double pivinv = 1.0 / c.s;
         done |= 1ull << icol;
-        // swap rows irow and icol, scale the new pivot row (gauss_errc.c:38-53); sh.col gets the
-        // pivot column of every other row as it stands after the swap
-        if (wave == 0 && l <= n) {
-            double *xr = (l < n) ? &a[irow * n + l] : &b[irow];
-            double *xc = (l < n) ? &a[icol * n + l] : &b[icol];
-            const double vr = *xr, vc = *xc;
-            *xc = ((l == icol) ? 1.0 : vr) * pivinv;
-            if (irow != icol) {
-                *xr = vc;
-                if (l == icol) sh.col[irow] = vc;
-            }
+        // publish the new pivot row (old row irow scaled, gauss_errc.c:44-53) and, for a swap, old row
+        // icol, which moves to row irow (:38-42)
+        if (wave == irow % kWaves) {
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < kGJRows; ++k)
+                if (k == irow / kWaves) v = row[k];
+            if (l <= n) sh.prow[l] = ((l == icol) ? 1.0 : v) * pivinv;
         }
-        for (int ll = tid; ll < n; ll += kWG)
-            if (ll != icol && ll != irow) sh.col[ll] = a[ll * n + icol];
+        if (irow != icol && wave == icol % kWaves) {
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < kGJRows; ++k)
+                if (k == icol / kWaves) v = row[k];
+            if (l <= n) sh.srow[l] = v;
+        }
         __syncthreads();
+        const double pr = (l <= n) ? sh.prow[l] : 0.0;
+        const double moved = (irow != icol && l <= n) ? sh.srow[l] : 0.0;
         // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
         c = {-1.0, 0.0, -1};
-        if (l <= n) {
-            const double pr = (l < n) ? a[icol * n + l] : b[icol];
-            const bool cand_col = l < n && !((done >> l) & 1);
-            for (int ll = wave; ll < n; ll += kWaves) {
-                if (ll == icol) continue;
-                const double dum = sh.col[ll];
-                if (l < n) {
-                    const double v = ((l == icol) ? 0.0 : a[ll * n + l]) - pr * dum;
-                    a[ll * n + l] = v;
-                    if (cand_col && !((done >> ll) & 1)) pivot_take(c, v, ll * n + l);
+        const bool cand_col = l < n && !((done >> l) & 1);
+#pragma unroll
+        for (int k = 0; k < kGJRows; ++k) {
+            const int ll = wave + kWaves * k;
+            const double cur = (ll == irow) ? moved : row[k];  // after the swap
+            const double dum = __shfl(cur, icol);               // every lane of the wave takes part
+            const double v = ((l == icol) ? 0.0 : cur) - pr * dum;
+            if (ll < n) {
+                if (ll == icol) {
+                    row[k] = pr;
                 } else {
-                    b[ll] -= pr * dum;
+                    row[k] = (l <= n) ? v : 0.0;
+                    if (cand_col && !((done >> ll) & 1)) pivot_take(c, v, ll * n + l);
                 }
             }
         }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kGJRows; ++k) {
+        const int ll = wave + kWaves * k;
+        if (ll < n && l <= n) {
+            if (l < n)
+                a[ll * n + l] = row[k];
+            else
+                b[ll] = row[k];
+        }
     }
     return 0;
 }
@@ -819,22 +844,22 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
                     ochisq = chi;
                     chisq = chi;  // a failed first step then reports this fit's own starting chi2
                     if (tid < ma) sh.atry[tid] = sh.cg[tid];
-                    __syncthreads();
-                    continue;
-                }
-                chisq = chi;
-                if (chisq < ochisq) {
-                    alamda = alamda / P.alamda_incr;
-                    cur ^= 1;
-                    if (tid < ma) sh.cg[tid] = sh.atry[tid];
                 } else {
-                    alamda = P.alamda_incr * alamda;
-                    chisq = ochisq;
+                    chisq = chi;
+                    if (chisq < ochisq) {
+                        alamda = alamda / P.alamda_incr;
+                        cur ^= 1;
+                        if (tid < ma) sh.cg[tid] = sh.atry[tid];
+                    } else {
+                        alamda = P.alamda_incr * alamda;
+                        chisq = ochisq;
+                    }
                 }
                 __syncthreads();
             } else {
                 ++nfail;
             }
+            if (init) continue;  // the starting point is evaluated; the first step follows
             ++niter;
             if (niter == 1 || chisq < ochisq) {
                 ochisq = chisq;
