@@ -155,6 +155,10 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
                        "GPU kernels support up to %d fitted of %d parameters (got %d of %d; Lmax <= 6)", kMaxFit,
                        psk::kRes, mfit, ma);
     if (geo->nfac > kMaxFac) return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support nrows <= 8 (nfac <= %d)", kMaxFac);
+    // the per-lightcurve normalisation table (nlc x (ma + 1) sums + nlc factors) lives in LDS
+    if ((in->lcurves + 1) * (ma + 3) > psk::kDsphLds)
+        return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support up to %d lightcurves at ma = %d", psk::kDsphLds / (ma + 3) - 1,
+                       ma);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ps_fail(PS_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return ps_fail(PS_ERR_ARG, "device %d out of range", device);
@@ -164,7 +168,6 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     h->device = device;
     h->ndata = in->ndata;
     h->npts = in->ndata + 3;
-    h->nlc = in->lcurves + 1;
     h->nlc = in->lcurves + 1;
     h->nfac = geo->nfac;
     h->ncoef = ncoef;
@@ -212,6 +215,27 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     lc_inrel[h->nlc - 1] = 0;
     for (int L = 0; L < h->nlc; ++L)
         for (int p = lc_start[L]; p < lc_start[L + 1]; ++p) lc_of[p] = L;
+    // runs of one lightcurve within the 16-point tiles of the model pass (segments); two guard entries
+    // at the end of seg_first cover the tile index one past the last
+    const int ntiles = (nobs + 15) / 16;
+    std::vector<int> seg_first(ntiles + 2), seg_lc, seg_range, lc_seg(h->nlc + 1);
+    for (int tl = 0; tl < ntiles; ++tl) {
+        seg_first[tl] = int(seg_lc.size());
+        const int p0 = 16 * tl, p1 = std::min(nobs, p0 + 16);
+        for (int p = p0; p < p1; ++p)
+            if (p == p0 || lc_of[p] != lc_of[p - 1]) {
+                if (p != p0) seg_range.back() |= (p - p0) << 8;
+                seg_lc.push_back(lc_of[p]);
+                seg_range.push_back(p - p0);
+            }
+        seg_range.back() |= (p1 - p0) << 8;
+    }
+    const int nseg = int(seg_lc.size());
+    seg_first[ntiles] = seg_first[ntiles + 1] = nseg;
+    for (int L = 0, s = 0; L <= h->nlc; ++L) {
+        while (s < nseg && seg_lc[s] < L) ++s;
+        lc_seg[L] = s;
+    }
     std::vector<double> nx(nf), ny(nf), nz(nf);
     for (int i = 0; i < nf; ++i) {
         nx[i] = geo->nor[3 * i];
@@ -276,6 +300,11 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.lc_start = dev_upload(lc_start.data(), lc_start.size(), o);
     P.lc_inrel = dev_upload(lc_inrel.data(), lc_inrel.size(), o);
     P.lc_of = dev_upload(lc_of.data(), lc_of.size(), o);
+    P.nseg = nseg;
+    P.seg_first = dev_upload(seg_first.data(), seg_first.size(), o);
+    P.seg_lc = dev_upload(seg_lc.data(), seg_lc.size(), o);
+    P.seg_range = dev_upload(seg_range.data(), seg_range.size(), o);
+    P.lc_seg = dev_upload(lc_seg.data(), lc_seg.size(), o);
     P.nx = dev_upload(nx.data(), nf, o);
     P.ny = dev_upload(ny.data(), nf, o);
     P.nz = dev_upload(nz.data(), nf, o);
@@ -313,7 +342,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     Slots &S = h->S;
     S.j_stride = size_t(npts) * psk::kMap;
     S.y_stride = size_t(npts);
-    S.dave_stride = size_t(h->nlc) * (ma + 1);
+    S.dave_stride = size_t(nseg) * psk::kMap;  // per-segment partial column sums
     S.a_stride = size_t(mfit) * mfit;
     S.b_stride = size_t(mfit);
     const size_t ns = size_t(h->slots);

@@ -53,6 +53,12 @@ struct DevProblem {
     double par1, par2, par3, log_cl;
     const double *t, *ex, *ey, *ez, *sx, *sy, *sz, *y, *s2i, *w, *sig;
     const int *lc_start, *lc_inrel, *lc_of;
+    // phase B tiles (16 observed points each) split into runs of one lightcurve ("segments"):
+    // tile t has segments seg_first[t] .. seg_first[t+1]-1, segment s covers tile-relative points
+    // seg_range[s] & 255 .. (seg_range[s] >> 8) - 1 of lightcurve seg_lc[s]; lightcurve L owns the
+    // consecutive segments lc_seg[L] .. lc_seg[L+1]-1
+    int nseg;
+    const int *seg_first, *seg_lc, *seg_range, *lc_seg;
     const double *nx, *ny, *nz, *darea;
     const double *dsph;  // [nf][dpitch], zero-padded beyond nc
     const float *dsphf;  // the same in FP32 (mixed precision)
@@ -292,10 +298,11 @@ struct TileSums {
 };
 
 template <int NT, int R, bool LDSB, bool MIX>
-__device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp) {
+__device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp, double *part) {
     using Pr = Prec<MIX>;
     using T = typename Pr::T;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index, provably uniform: loops over it hold cross-lane shuffles
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int pl = lane & 15, kg = lane >> 4;
     const int nc = P.nc, pitch = P.dpitch;
     const double *a = sh.a;
@@ -379,23 +386,43 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
             const int tile = kTP * tp + h;
+            const int s0 = P.seg_first[tile], s1 = P.seg_first[tile + 1];
+            double val[NT][4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = Pr::row(r, kg);
                 const double sc = __shfl(scale[h], i);
                 const int pi = tile * 16 + i;
-                if (pi < P.nobs) {
-                    double *jr = J + size_t(pi) * kMap;
+                double *jr = J + size_t(pi < P.nobs ? pi : 0) * kMap;
 #pragma unroll
-                    for (int tt = 0; tt < NT; ++tt)
-                        if (16 * tt + pl < nc) jr[16 * tt + pl] = sc * double(ts[h].acc[tt][r]);
+                for (int tt = 0; tt < NT; ++tt) {
+                    val[tt][r] = (pi < P.nobs) ? sc * double(ts[h].acc[tt][r]) : 0.0;
+                    if (pi < P.nobs && 16 * tt + pl < nc) jr[16 * tt + pl] = val[tt][r];
+                }
+            }
+            // per-lightcurve column sums of this tile's rows, one partial row per segment
+            for (int sg = s0; sg < s1; ++sg) {
+                if (!P.lc_inrel[P.seg_lc[sg]]) continue;
+                const int q0 = P.seg_range[sg] & 255, q1 = P.seg_range[sg] >> 8;
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) {
+                    double x = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = Pr::row(r, kg);
+                        x += (i >= q0 && i < q1) ? val[tt][r] : 0.0;
+                    }
+                    x += __shfl_xor(x, 16);
+                    x += __shfl_xor(x, 32);
+                    if (kg == 0 && 16 * tt + pl < nc) part[size_t(sg) * kMap + 16 * tt + pl] = x;
                 }
             }
         }
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
             TileSums<NT, R, MIX> &s = ts[h];
-            const int p = (kTP * tp + h) * 16 + pl;
+            const int tile = kTP * tp + h;
+            const int p = tile * 16 + pl;
             const double br = sum_groups(s.br);
             double S[3], U[3];
 #pragma unroll
@@ -407,20 +434,44 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
             T racc[R > 0 ? R : 1];
 #pragma unroll
             for (int r = 0; r < R; ++r) racc[r] = sum_groups(s.racc[r]);
-            if (kg == 0 && valid[h]) {
+            // the remaining columns of point p (every lane group holds the same point-p values)
+            constexpr int kOther = R + 9;  // R leftover shape columns, beta/lambda/omega, 3 phase, cl, cls, ymod
+            double ov[kOther];
+            {
                 double js[3];
-                spin_columns(P, a, p, sh, S, U, js);
+                spin_columns(P, a, valid[h] ? p : 0, sh, S, U, js);
+#pragma unroll
+                for (int r = 0; r < R; ++r) ov[r] = scale[h] * double(racc[r]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ov[R + k] = scale[h] * js[k];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) ov[R + 3 + i] = br * dphp[h][i];
+                ov[R + 6] = scale[h] * d1 * cl_d;
+                ov[R + 7] = scale[h] * d2;
+                ov[R + 8] = br * scale[h];
+            }
+            auto ocol = [&](int k) { return (k < R) ? 16 * NT + k : (k < R + 8) ? nc + (k - R) : P.ma; };
+            if (kg == 0 && valid[h]) {
                 double *jr = J + size_t(p) * kMap;
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (16 * NT + r < nc) jr[16 * NT + r] = scale[h] * double(racc[r]);
+                for (int k = 0; k < kOther - 1; ++k)
+                    if (k >= R || 16 * NT + k < nc) jr[ocol(k)] = ov[k];
+                ytemp[p] = ov[kOther - 1];
+            }
+            for (int sg = P.seg_first[tile]; sg < P.seg_first[tile + 1]; ++sg) {
+                if (!P.lc_inrel[P.seg_lc[sg]]) continue;
+                const int q0 = P.seg_range[sg] & 255, q1 = P.seg_range[sg] >> 8;
+                const bool in = valid[h] && pl >= q0 && pl < q1;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) jr[nc + k] = scale[h] * js[k];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) jr[nc + 3 + i] = br * dphp[h][i];
-                jr[nc + 6] = scale[h] * d1 * cl_d;
-                jr[nc + 7] = scale[h] * d2;
-                ytemp[p] = br * scale[h];
+                for (int k = 0; k < kOther; ++k) {
+                    if (k < R && 16 * NT + k >= nc) continue;
+                    double x = in ? ov[k] : 0.0;
+                    x += __shfl_xor(x, 1);
+                    x += __shfl_xor(x, 2);
+                    x += __shfl_xor(x, 4);
+                    x += __shfl_xor(x, 8);
+                    if (lane == 0) part[size_t(sg) * kMap + ocol(k)] = x;
+                }
             }
         }
     }
@@ -455,7 +506,7 @@ template <int NT, int R, bool LDSB, bool MIX>
 __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *dave, double *alpha, double *beta,
                          Shared &sh, double *raw_y, double *raw_dyda) {
     using Pd = Prec<false>;  // phase D arithmetic
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int pl = lane & 15, kg = lane >> 4;
     const int ma = P.ma, npts = P.npts, mfit = P.mfit;
     curv_lds(P, sh.a, sh);
@@ -473,7 +524,7 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
             sh.aclsf[i] = float(sh.acls[i]);
         }
     __syncthreads();
-    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp);
+    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp, dave);
     __syncthreads();
     if (raw_y) {  // raw bright()/conv() rows, before the lightcurve normalisation below
         for (int p = tid; p < npts; p += kWG) raw_y[p] = ytemp[p];
@@ -482,60 +533,39 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
             raw_dyda[it] = J[size_t(p) * kMap + k];
         }
     }
-    // Phase C: relative-lightcurve normalisation in place (mrqcof.c:53-104).  One wave per lightcurve,
-    // lanes on columns (coalesced rows): the column sums dave_l = sum dyda_l and ave = sum ymod over
-    // the curve's points, then each row becomes coef*(dyda_l - ymod*dave_l/ave) with
-    // coef = sig*Lpoints/ave, dyda for c00 = 0, and the residual y - coef*ymod goes to column kRes.
-    // Calibrated curves and the convexity rows keep their rows; their residual is y - ymod.
-    for (int L = wave; L <= P.nlc; L += kWaves) {
-        const int p0 = (L < P.nlc) ? P.lc_start[L] : P.nobs, p1 = (L < P.nlc) ? P.lc_start[L + 1] : npts;
-        if (L < P.nlc && P.lc_inrel[L]) {
-            const int l = lane;  // ma + 1 <= 64 columns (checked by ps_prepare)
-            const double *src = (l < ma) ? J + l : ytemp;
-            const size_t st = (l < ma) ? kMap : 1;
-            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            if (l <= ma) {
-                int p = p0;
-                for (; p + 4 <= p1; p += 4) {
-                    s0 += src[size_t(p) * st];
-                    s1 += src[size_t(p + 1) * st];
-                    s2 += src[size_t(p + 2) * st];
-                    s3 += src[size_t(p + 3) * st];
-                }
-                for (; p < p1; ++p) s0 += src[size_t(p) * st];
-            }
-            const double dv = (s0 + s1) + (s2 + s3);
-            const double ave = __shfl(dv, ma);
-            const double r = dv / ave;
-            const double cpts = double(p1 - p0) / ave;
-            // rows in groups of four: all loads of a group are issued before its stores
-            for (int pb = p0; pb < p1; pb += 4) {
-                double yt[4], sg[4], jv[4], yo[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int pq = (pb + q < p1) ? pb + q : pb;
-                    yt[q] = ytemp[pq];
-                    sg[q] = P.sig[pq];
-                    jv[q] = (l < ma) ? J[size_t(pq) * kMap + l] : 0.0;
-                    yo[q] = (l == kRes) ? P.y[pq] : 0.0;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (pb + q >= p1) break;
-                    const double coef = sg[q] * cpts;
-                    double *jr = J + size_t(pb + q) * kMap;
-                    if (l < ma) jr[l] = (l == 0) ? 0.0 : coef * (jv[q] - yt[q] * r);
-                    if (l == kRes) jr[kRes] = yo[q] - coef * yt[q];
-                }
-            }
+    // Phase C: per relative lightcurve L the column sums dave_l = sum dyda_l and ave = sum ymod over
+    // its points (mrqcof.c:53-87), added up from the per-segment partial rows of phase B, into an LDS
+    // table in the union (Dsph is no longer needed): tab[L][l] = dave_l / ave for l < ma,
+    // tab[L][ma] = ave, cfl[L] = Lpoints / ave and rell[L] = 1 for relative curves (ave, and so cfl, can
+    // have either sign: a trial step may turn the phase function negative).
+    const int tpitch = ma + 1;
+    double *tab = sh.u.dsph, *cfl = sh.u.dsph + P.nlc * tpitch;
+    int *rell = reinterpret_cast<int *>(cfl + P.nlc);
+    for (int it = tid; it < P.nlc * tpitch; it += kWG) {
+        const int L = it / tpitch, l = it - L * tpitch;
+        double s = 0;
+        if (P.lc_inrel[L])
+            for (int q = P.lc_seg[L]; q < P.lc_seg[L + 1]; ++q) s += dave[size_t(q) * kMap + (l < ma ? l : ma)];
+        tab[it] = s;
+    }
+    __syncthreads();
+    for (int it = tid; it < P.nlc * tpitch; it += kWG) {
+        const int L = it / tpitch, l = it - L * tpitch;
+        const bool rel = P.lc_inrel[L];
+        if (l < ma) {
+            if (rel) tab[it] = tab[it] / tab[L * tpitch + ma];
         } else {
-            for (int p = p0 + lane; p < p1; p += 64) J[size_t(p) * kMap + kRes] = P.y[p] - ytemp[p];
+            cfl[L] = rel ? double(P.lc_start[L + 1] - P.lc_start[L]) / tab[it] : 0.0;
+            rell[L] = rel;
         }
     }
     __syncthreads();
-    // Phase D: alpha/beta/chi2 (mrqcof.c:106-145).  Each wave takes 4-point K-steps; lane (pl, kg) holds
+    // Phase D: alpha/beta/chi2 (mrqcof.c:89-145).  Each wave takes 4-point K-steps; lane (pl, kg) holds
     // row 4s + kg, column 16u + pl of the normalised fitted columns plus the residual as column mfit,
-    // which is at once the MFMA A operand (column x point) and, times w/sig^2, the B operand.
+    // which is at once the MFMA A operand (column x point) and, times w/sig^2, the B operand.  A row
+    // of relative lightcurve L is normalised as it is loaded: coef*(dyda_l - ymod*dave_l/ave) with
+    // coef = sig*Lpoints/ave, dyda for c00 = 0, residual y - coef*ymod; other rows keep dyda and have
+    // residual y - ymod.
     const int nb = (mfit + 16) >> 4;  // 16-column blocks covering mfit + 1 columns
     int colu[4];
 #pragma unroll
@@ -547,29 +577,63 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
 #pragma unroll
     for (int b = 0; b < 10; ++b) acc[b] = Pd::zero();
     const int nsteps = (npts + 3) >> 2;
-    // row loads of step s: four columns of point 4s + kg and its weight w/sig^2
-    auto load_step = [&](int s, double v[4], double &wq) {
-        const int p = 4 * s + kg;
-        const bool in = p < npts;
-        const double *jr = J + size_t(in ? p : 0) * kMap;
-        wq = in ? P.s2i[p] * P.w[p] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = (u < nb && colu[u] >= 0) ? jr[colu[u]] : 0.0;
+    // Two-stage software pipeline over the K-steps: the global loads of step s + 2 (raw columns and
+    // the point's weight, model value, observation and lightcurve) are in flight while step s + 1 is
+    // normalised from the LDS table and step s runs on the MFMA.
+    struct RowG {
+        double raw[4], wq, yt, sig, y;
+        int L;
+        bool in;
     };
-    double vcur[4], wcur;
-    if (wave < nsteps) load_step(wave, vcur, wcur);
+    struct RowV {
+        double v[4], wq;
+    };
+    auto stage1 = [&](int s, RowG &g) {
+        const int p = 4 * s + kg;
+        g.in = s < nsteps && p < npts;
+        const int pp = g.in ? p : 0;
+        const double *jr = J + size_t(pp) * kMap;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g.raw[u] = (u < nb && colu[u] >= 0 && colu[u] != kRes) ? jr[colu[u]] : 0.0;
+        g.wq = g.in ? P.s2i[pp] * P.w[pp] : 0.0;
+        g.yt = ytemp[pp];
+        g.sig = P.sig[pp];
+        g.y = P.y[pp];
+        g.L = P.lc_of[pp];
+    };
+    auto stage2 = [&](const RowG &g, RowV &r) {
+        const bool rel = rell[g.L];
+        const double coef = rel ? g.sig * cfl[g.L] : 1.0;
+        const double *m = tab + g.L * tpitch;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int col = colu[u];
+            double x = 0;
+            if (u < nb && col >= 0)
+                x = (col == kRes) ? g.y - coef * g.yt
+                                  : (rel ? ((col == 0) ? 0.0 : coef * (g.raw[u] - g.yt * m[col])) : g.raw[u]);
+            r.v[u] = g.in ? x : 0.0;
+        }
+        r.wq = g.wq;
+    };
+    RowG g1, g2;
+    RowV cur;
+    stage1(wave, g1);
+    stage1(wave + kWaves, g2);
+    stage2(g1, cur);
     for (int s = wave; s < nsteps; s += kWaves) {
-        double vnext[4], wnext = 0;
-        if (s + kWaves < nsteps) load_step(s + kWaves, vnext, wnext);  // in flight during the MFMAs below
+        RowG g3;
+        stage1(s + 2 * kWaves, g3);
+        RowV nxt;
+        stage2(g2, nxt);
         int idx = 0;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj <= bi; ++bj, ++idx)  // blocks beyond nb add zeros
-                acc[idx] = Pd::mma(vcur[bi], wcur * vcur[bj], acc[idx]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) vcur[u] = vnext[u];
-        wcur = wnext;
+                acc[idx] = Pd::mma(cur.v[bi], cur.wq * cur.v[bj], acc[idx]);
+        cur = nxt;
+        g2 = g3;
     }
     __syncthreads();
     // deterministic cross-wave sum into u.d.red (lower blocks; diagonal blocks in full)
