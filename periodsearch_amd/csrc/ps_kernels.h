@@ -91,6 +91,7 @@ struct Shared {
     double scal[8];
     int iscal[4];
     double bl[9], dbl[18];  // pole rotation and its beta/lambda derivatives (blmatrix)
+    double hbl[18];         // dbl_j * bl^T: the beta/lambda derivative generators in the pole frame
     union {
         double dsph[kDsphLds];       // phase B, FP64
         float dsphf[2 * kDsphLds];   // phase B, FP32
@@ -184,6 +185,10 @@ __device__ inline void blmatrix_lds(double bet, double lam, Shared &sh) {
         blmatrix(bet, lam, bl, dbl);
         for (int i = 0; i < 9; ++i) sh.bl[i] = bl[i / 3][i % 3];
         for (int i = 0; i < 18; ++i) sh.dbl[i] = dbl[i / 9][(i / 3) % 3][i % 3];
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k)
+                    sh.hbl[9 * j + 3 * i + k] = dbl[j][i][0] * bl[k][0] + dbl[j][i][1] * bl[k][1] + dbl[j][i][2] * bl[k][2];
     }
 }
 
@@ -196,11 +201,14 @@ __device__ inline void rotation_phase(double omg, double t, double phi0, double 
 
 // Earth and Sun directions of point p in the body frame (bright.c:41-57 with matrix.c).
 __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, const Shared &sh, double e[3],
-                                  double e0[3]) {
+                                  double e0[3], double rot[3]) {
     const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
     const double ee0[3] = {P.sx[p], P.sy[p], P.sz[p]};
     double sf, cf;
-    rotation_phase(a[P.nc + 2], P.t[p], P.phi0, sf, cf);
+    rot[2] = P.t[p];
+    rotation_phase(a[P.nc + 2], rot[2], P.phi0, sf, cf);
+    rot[0] = sf;
+    rot[1] = cf;
     const double fm[3][3] = {{cf, sf, 0}, {-sf, cf, 0}, {0, 0, 1}};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -218,12 +226,11 @@ __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, c
     }
 }
 
-// Phase function of point p and its derivatives w.r.t. the three phase parameters
-// (phasec.c:11-24, bright.c:30-35).
-__device__ inline double phase_scale(const DevProblem &P, const double *a, int p, double dphp[3]) {
-    const int nc = P.nc;
-    const double cos_alpha = P.ex[p] * P.sx[p] + P.ey[p] * P.sy[p] + P.ez[p] * P.sz[p];
-    const double alpha = acos(cos_alpha);
+// Phase function of a point and its derivatives w.r.t. the three phase parameters (phasec.c:11-24,
+// bright.c:30-35).  The phase angle comes from the body-frame directions: cos alpha = e.e0 is
+// invariant under the common rotation.
+__device__ inline double phase_scale(const double *a, int nc, const double e[3], const double e0[3], double dphp[3]) {
+    const double alpha = acos(e[0] * e0[0] + e[1] * e0[1] + e[2] * e0[2]);
     const double p1 = a[nc + 3], p2 = a[nc + 4], p3 = a[nc + 5];
     const double ephi = exp(-alpha / p2);
     dphp[0] = ephi;
@@ -232,39 +239,29 @@ __device__ inline double phase_scale(const DevProblem &P, const double *a, int p
     return 1 + p1 * ephi + p3 * alpha;
 }
 
-// Brightness derivatives w.r.t. beta, lambda, omega before the phase factor (bright.c:88-104):
-// js[j] = sum_i S[i]*de[i][j] + U[i]*de0[i][j], where de / de0 are the derivatives of the body-frame
-// Earth / Sun directions (matrix.c:11-58, blmatrix.c) and S, U the facet sums of dbr/de, dbr/de0.
-// Each de[i][j] is formed and consumed in turn, never held as a table.
-__device__ inline void spin_columns(const DevProblem &P, const double *a, int p, const Shared &sh, const double S[3],
-                                    const double U[3], double js[3]) {
-    const int nc = P.nc;
-    const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
-    const double ee0[3] = {P.sx[p], P.sy[p], P.sz[p]};
-    const double t = P.t[p];
-    double sf, cf;
-    rotation_phase(a[nc + 2], t, P.phi0, sf, cf);
-    const double fm[3][3] = {{cf, sf, 0}, {-sf, cf, 0}, {0, 0, 1}};
-    const double dfm[3][3] = {{-t * sf, t * cf, 0}, {-t * cf, -t * sf, 0}, {0, 0, 0}};
+// Brightness derivatives w.r.t. beta, lambda, omega before the phase factor (bright.c:88-104 with
+// matrix.c:11-58): js[j] = S.de_j + U.de0_j, S and U the facet sums of dbr/de and dbr/de0.  With the
+// body-frame direction e = Rz(phi) B ee: de_j = Rz H_j Rz^T e for beta and lambda (H_j = dB_j B^T) and
+// de_omega = t (e_y, -e_x, 0) (dRz/dphi Rz^T is that generator); rot = (sin phi, cos phi, t).
+__device__ inline void spin_columns(const Shared &sh, const double e[3], const double e0[3], const double rot[3],
+                                    const double S[3], const double U[3], double js[3]) {
+    const double sf = rot[0], cf = rot[1], t = rot[2];
+    const double ep[3] = {cf * e[0] - sf * e[1], sf * e[0] + cf * e[1], e[2]};  // Rz^T e
+    const double e0p[3] = {cf * e0[0] - sf * e0[1], sf * e0[0] + cf * e0[1], e0[2]};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        double acc = 0;
+    for (int j = 0; j < 2; ++j) {
+        const double *H = sh.hbl + 9 * j;
+        double h[3], h0[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {  // de[i][j] = sum_k dtm[j][i][k] ee[k]
-            double d = 0, d0 = 0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                double dt = 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    dt = dt + (j < 2 ? fm[i][q] * sh.dbl[9 * j + 3 * q + k] : dfm[i][q] * sh.bl[3 * q + k]);
-                d = d + dt * ee[k];
-                d0 = d0 + dt * ee0[k];
-            }
-            acc = acc + (S[i] * d + U[i] * d0);
+        for (int i = 0; i < 3; ++i) {
+            h[i] = H[3 * i] * ep[0] + H[3 * i + 1] * ep[1] + H[3 * i + 2] * ep[2];
+            h0[i] = H[3 * i] * e0p[0] + H[3 * i + 1] * e0p[1] + H[3 * i + 2] * e0p[2];
         }
-        js[j] = acc;
+        const double d[3] = {cf * h[0] + sf * h[1], -sf * h[0] + cf * h[1], h[2]};
+        const double d0[3] = {cf * h0[0] + sf * h0[1], -sf * h0[0] + cf * h0[1], h0[2]};
+        js[j] = (S[0] * d[0] + S[1] * d[1] + S[2] * d[2]) + (U[0] * d0[0] + U[1] * d0[1] + U[2] * d0[2]);
     }
+    js[2] = t * ((S[0] * e[1] - S[1] * e[0]) + (U[0] * e0[1] - U[1] * e0[0]));
 }
 
 template <class T>
@@ -319,18 +316,24 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
     const int nchunk = P.nf >> 2;  // nf = 8 nrows^2
     for (int tp = wave; kTP * tp < ntiles; tp += kWaves) {
         T e[kTP][3], e0[kTP][3];
+        double rot[kTP][3];                 // sin phi, cos phi, t: all the epilogue needs of the point
+        double ed[kTP][MIX ? 3 : 1], e0d[kTP][MIX ? 3 : 1];  // FP64 directions when e, e0 are FP32
         bool valid[kTP];
         TileSums<NT, R, MIX> ts[kTP];
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
             const int p = (kTP * tp + h) * 16 + pl;
             valid[h] = p < P.nobs;
-            double ed[3], e0d[3];
-            point_dirs(P, a, valid[h] ? p : 0, sh, ed, e0d);
+            double e64[3], e064[3];
+            point_dirs(P, a, valid[h] ? p : 0, sh, e64, e064, rot[h]);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                e[h][k] = T(ed[k]);
-                e0[h][k] = T(e0d[k]);
+                e[h][k] = T(e64[k]);
+                e0[h][k] = T(e064[k]);
+                if constexpr (MIX) {
+                    ed[h][k] = e64[k];
+                    e0d[h][k] = e064[k];
+                }
                 ts[h].S[k] = 0;
                 ts[h].U[k] = 0;
             }
@@ -377,11 +380,25 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
         // Epilogue.  The phase factors of both tiles first, so that the MFMA accumulators (the shape
         // columns: register r of lane l holds point Pr::row(r, kg), coefficient 16tt + pl) are written
         // and released before the spin columns are formed.
+        // the body-frame directions in FP64
+        auto dir = [&](int h, double v[3], double v0[3]) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if constexpr (MIX) {
+                    v[k] = ed[h][k];
+                    v0[k] = e0d[h][k];
+                } else {
+                    v[k] = e[h][k];
+                    v0[k] = e0[h][k];
+                }
+            }
+        };
         double scale[kTP], dphp[kTP][3];
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
-            const int p = (kTP * tp + h) * 16 + pl;
-            scale[h] = phase_scale(P, a, valid[h] ? p : 0, dphp[h]);
+            double v[3], v0[3];
+            dir(h, v, v0);
+            scale[h] = phase_scale(a, nc, v, v0, dphp[h]);
         }
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
@@ -438,8 +455,9 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
             constexpr int kOther = R + 9;  // R leftover shape columns, beta/lambda/omega, 3 phase, cl, cls, ymod
             double ov[kOther];
             {
-                double js[3];
-                spin_columns(P, a, valid[h] ? p : 0, sh, S, U, js);
+                double js[3], v[3], v0[3];
+                dir(h, v, v0);
+                spin_columns(sh, v, v0, rot[h], S, U, js);
 #pragma unroll
                 for (int r = 0; r < R; ++r) ov[r] = scale[h] * double(racc[r]);
 #pragma unroll
