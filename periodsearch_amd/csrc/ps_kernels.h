@@ -264,11 +264,56 @@ __device__ inline void spin_columns(const Shared &sh, const double e[3], const d
     js[2] = t * ((S[0] * e[1] - S[1] * e[0]) + (U[0] * e0[1] - U[1] * e0[0]));
 }
 
+// A wave-uniform read of read-only problem data through the scalar cache (s_load): unlike a vector
+// load it does not wait behind the wave's outstanding vector stores.
+__device__ inline int uload(const int *p, int i) {
+    return reinterpret_cast<const __attribute__((address_space(4))) int *>(reinterpret_cast<unsigned long long>(p))[i];
+}
+
+// Cross-lane sums on the VALU (no LDS round trip, no index registers).  v_permlane16/32_swap give each
+// lane the pair (x[l & ~16], x[l | 16]) (resp. 32), added in that fixed order, so both lanes of a
+// pair get the same bits; DPP quad permutes and row (half) mirrors pair lanes within a 16-lane row.
+__device__ inline double pair_sum(double x, bool rows32) {
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(b), hi = unsigned(b >> 32);
+    unsigned a0, a1, h0, h1;
+    if (rows32) {
+        auto l2 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        auto h2 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        a0 = l2[0]; a1 = l2[1]; h0 = h2[0]; h1 = h2[1];
+    } else {
+        auto l2 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        auto h2 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        a0 = l2[0]; a1 = l2[1]; h0 = h2[0]; h1 = h2[1];
+    }
+    return __longlong_as_double((long long)((unsigned long long)h0 << 32 | a0)) +
+           __longlong_as_double((long long)((unsigned long long)h1 << 32 | a1));
+}
+__device__ inline float pair_sum(float x, bool rows32) {
+    const unsigned u = __float_as_uint(x);
+    auto r = rows32 ? __builtin_amdgcn_permlane32_swap(u, u, false, false) : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 template <class T>
 __device__ inline T sum_groups(T v) {  // sum over the four 16-lane groups of a wave
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+    return pair_sum(pair_sum(v, false), true);
+}
+
+template <int CTRL>
+__device__ inline double dpp(double x) {
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_mov_dpp(unsigned(b), CTRL, 0xF, 0xF, false);
+    const unsigned hi = __builtin_amdgcn_mov_dpp(unsigned(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
+
+__device__ inline double sum_row16(double x) {  // sum over the lane's 16-lane row
+    x += dpp<0xB1>(x);   // quad_perm [1,0,3,2]: lane ^ 1
+    x += dpp<0x4E>(x);   // quad_perm [2,3,0,1]: lane ^ 2
+    x += dpp<0x141>(x);  // row_half_mirror: the other quad of the 8
+    x += dpp<0x140>(x);  // row_mirror: the other 8 of the row
+    return x;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -403,7 +448,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
             const int tile = kTP * tp + h;
-            const int s0 = P.seg_first[tile], s1 = P.seg_first[tile + 1];
+            const int s0 = uload(P.seg_first, tile), s1 = uload(P.seg_first, tile + 1);
             double val[NT][4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -419,8 +464,8 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
             }
             // per-lightcurve column sums of this tile's rows, one partial row per segment
             for (int sg = s0; sg < s1; ++sg) {
-                if (!P.lc_inrel[P.seg_lc[sg]]) continue;
-                const int q0 = P.seg_range[sg] & 255, q1 = P.seg_range[sg] >> 8;
+                if (!uload(P.lc_inrel, uload(P.seg_lc, sg))) continue;
+                const int rg = uload(P.seg_range, sg), q0 = rg & 255, q1 = rg >> 8;
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) {
                     double x = 0;
@@ -429,8 +474,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                         const int i = Pr::row(r, kg);
                         x += (i >= q0 && i < q1) ? val[tt][r] : 0.0;
                     }
-                    x += __shfl_xor(x, 16);
-                    x += __shfl_xor(x, 32);
+                    x = sum_groups(x);
                     if (kg == 0 && 16 * tt + pl < nc) part[size_t(sg) * kMap + 16 * tt + pl] = x;
                 }
             }
@@ -476,18 +520,14 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                     if (k >= R || 16 * NT + k < nc) jr[ocol(k)] = ov[k];
                 ytemp[p] = ov[kOther - 1];
             }
-            for (int sg = P.seg_first[tile]; sg < P.seg_first[tile + 1]; ++sg) {
-                if (!P.lc_inrel[P.seg_lc[sg]]) continue;
-                const int q0 = P.seg_range[sg] & 255, q1 = P.seg_range[sg] >> 8;
+            for (int sg = uload(P.seg_first, tile); sg < uload(P.seg_first, tile + 1); ++sg) {
+                if (!uload(P.lc_inrel, uload(P.seg_lc, sg))) continue;
+                const int rg = uload(P.seg_range, sg), q0 = rg & 255, q1 = rg >> 8;
                 const bool in = valid[h] && pl >= q0 && pl < q1;
 #pragma unroll
                 for (int k = 0; k < kOther; ++k) {
                     if (k < R && 16 * NT + k >= nc) continue;
-                    double x = in ? ov[k] : 0.0;
-                    x += __shfl_xor(x, 1);
-                    x += __shfl_xor(x, 2);
-                    x += __shfl_xor(x, 4);
-                    x += __shfl_xor(x, 8);
+                    const double x = sum_row16(in ? ov[k] : 0.0);
                     if (lane == 0) part[size_t(sg) * kMap + ocol(k)] = x;
                 }
             }
