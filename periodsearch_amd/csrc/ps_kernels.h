@@ -743,6 +743,48 @@ __device__ inline void pivot_take(PivotCand &c, double s, int q) {
     if (v > c.v || (v == c.v && q > c.q)) c = {v, s, q};
 }
 
+__device__ inline void pivot_merge(PivotCand &c, const PivotCand &o) {
+    if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
+}
+
+template <int CTRL>
+__device__ inline PivotCand pivot_dpp(const PivotCand &c) {
+    return {dpp<CTRL>(c.v), dpp<CTRL>(c.s), int(__builtin_amdgcn_mov_dpp(unsigned(c.q), CTRL, 0xF, 0xF, false))};
+}
+
+__device__ inline double swap_half(double x, bool rows32, int k) {  // element k of the permlane swap pair
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(b), hi = unsigned(b >> 32);
+    auto l2 = rows32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false) : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto h2 = rows32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false) : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __longlong_as_double((long long)((unsigned long long)h2[k] << 32 | l2[k]));
+}
+
+// Wave-wide argmax of the pivot candidates (a total order: larger |a|, then larger index), on DPP and
+// permlane swaps; every lane ends with the same candidate.
+__device__ inline PivotCand pivot_wave_max(PivotCand c) {
+    pivot_merge(c, pivot_dpp<0xB1>(c));
+    pivot_merge(c, pivot_dpp<0x4E>(c));
+    pivot_merge(c, pivot_dpp<0x141>(c));
+    pivot_merge(c, pivot_dpp<0x140>(c));
+#pragma unroll
+    for (int r32 = 0; r32 < 2; ++r32) {
+        const unsigned q = unsigned(c.q);
+        auto qq = r32 ? __builtin_amdgcn_permlane32_swap(q, q, false, false) : __builtin_amdgcn_permlane16_swap(q, q, false, false);
+        PivotCand a{swap_half(c.v, r32, 0), swap_half(c.s, r32, 0), int(qq[0])};
+        const PivotCand b{swap_half(c.v, r32, 1), swap_half(c.s, r32, 1), int(qq[1])};
+        pivot_merge(a, b);
+        c = a;
+    }
+    return c;
+}
+
+__device__ inline double read_lane(double x, int l) {  // x of lane l (l wave-uniform)
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(b), l), hi = __builtin_amdgcn_readlane(unsigned(b >> 32), l);
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
+
 constexpr int kGJRows = (kMaxFit + kWaves - 1) / kWaves;  // rows held per wave
 
 __device__ inline int gauss_jordan(Shared &sh, int n) {
@@ -759,11 +801,8 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
     }
     unsigned long long done = 0;  // pivoted rows / columns
     for (int i = 0; i < n; ++i) {
-        // workgroup argmax: waves by shuffles, then every lane combines the eight wave results
-        for (int off = 32; off > 0; off >>= 1) {
-            const PivotCand o{__shfl_xor(c.v, off), __shfl_xor(c.s, off), __shfl_xor(c.q, off)};
-            if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
-        }
+        // workgroup argmax: each wave on the VALU, then every lane combines the eight wave results
+        c = pivot_wave_max(c);
         if (l == 0) {
             sh.red_v[wave] = c.v;
             sh.red_s[wave] = c.s;
@@ -810,7 +849,7 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         for (int k = 0; k < kGJRows; ++k) {
             const int ll = wave + kWaves * k;
             const double cur = (ll == irow) ? moved : row[k];  // after the swap
-            const double dum = __shfl(cur, icol);               // every lane of the wave takes part
+            const double dum = read_lane(cur, icol);            // the pivot-column entry of row ll
             const double v = ((l == icol) ? 0.0 : cur) - pr * dum;
             if (ll < n) {
                 if (ll == icol) {
