@@ -7,6 +7,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "libps_oracle.so")
+ORACLE_FMA_SO = os.path.join(ROOT, "oracle", "libps_oracle_fma.so")  # the same restatement, FMA-contracted
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libps_ref.so")
 ORACLE_CLI = os.path.join(ROOT, "oracle", "ps_oracle_cli")
 
@@ -27,15 +28,17 @@ class _Fit(C.Structure):
         ("niter", C.c_int)]
 
 
-_o = None
+_libs = {}
 
 
-def oracle():
-    global _o
-    if _o is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError("oracle/libps_oracle.so missing: run make -C oracle")
-        o = C.CDLL(ORACLE_SO)
+def oracle(fma=False):
+    """The oracle library: the plain FP64 build, or (fma=True) the FMA-contracted build -- a second
+    valid FP64 rounding of the reference arithmetic."""
+    path = ORACLE_FMA_SO if fma else ORACLE_SO
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{os.path.relpath(path, ROOT)} missing: run make -C oracle")
+        o = C.CDLL(path)
         o.pso_ctx_from_file.restype = VP
         o.pso_ctx_from_file.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
         o.pso_ctx_from_text.restype = VP
@@ -78,8 +81,8 @@ def oracle():
         o.pso_ellfit.argtypes = [DP, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, DP, DP, DP, C.c_int,
                                  C.c_int]
         o.pso_ncoef.argtypes = [C.c_int, C.c_int]
-        _o = o
-    return _o
+        _libs[path] = o
+    return _libs[path]
 
 
 def ref_available():
@@ -89,8 +92,8 @@ def ref_available():
 class OracleCtx:
     """One parsed problem in the oracle (period_search_BOINC.cpp state, without globals)."""
 
-    def __init__(self, ptr):
-        self.o = oracle()
+    def __init__(self, ptr, fma=False):
+        self.o = oracle(fma)
         self.c = VP(ptr)
         self.ma = self.o.pso_ctx_ma(self.c)
         self.ncoef = self.o.pso_ctx_ncoef(self.c)
@@ -99,12 +102,12 @@ class OracleCtx:
         self.ndata = self.o.pso_ctx_ndata(self.c)
 
     @classmethod
-    def from_file(cls, path):
+    def from_file(cls, path, fma=False):
         err = C.c_int()
-        p = oracle().pso_ctx_from_file(str(path).encode(), C.byref(err))
+        p = oracle(fma).pso_ctx_from_file(str(path).encode(), C.byref(err))
         if not p:
             raise RuntimeError(f"oracle cannot load {path}: {err.value}")
-        return cls(p)
+        return cls(p, fma)
 
     @classmethod
     def from_text(cls, text):

@@ -310,3 +310,53 @@ def test_mixed_scan_same_best_period():
     b = ps.scan(1, 40, cw)
     ps.close()
     assert np.median(np.abs(b["chi2"] - a["chi2"]) / a["chi2"]) <= 1e-5
+
+
+# ---- the reference's real inputs (sparse surveys, Lmax = 3) and the process contract
+def _real(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "real_inputs", name)
+
+
+@pytest.mark.parametrize("name", ["period_search_in_20188", "period_search_in_23046", "period_search_in_2543"])
+def test_real_input_scan_matches_oracle(name):
+    """Three of the reference's own inputs (1, 4 and 19 lightcurves): the first two trial periods at the
+    input's initial convexity weight, GPU against the oracle."""
+    ps = PeriodSearch.from_file(_real(name)).prepare(device=0)
+    orc = ob.OracleCtx.from_file(_real(name))
+    cw = ps.info()["conw_r0"]
+    assert cw == orc.conw_r0
+    f0, _, df, nmax = orc.freq_grid()
+    inf = ps.info()
+    assert (inf["freq_start"], inf["freq_step"], inf["nmax"]) == (f0, df, nmax)
+    got = ps.scan(1, 2, cw)
+    ref = orc.scan(1, 2, cw)
+    ps.close()
+    g = np.stack([got[k] for k in ("per", "dev", "chi2", "dark", "la", "be")], 1)
+    # a line must match the oracle to 1e-9, or -- where the oracle's FMA-contracted build, a second
+    # valid FP64 rounding of the reference, differs from it by more -- lie within that spread
+    fma = ob.OracleCtx.from_file(_real(name), fma=True).scan(1, 2, cw)
+    for i in range(len(g)):
+        spread = np.abs(fma[i] - ref[i])
+        tol = np.concatenate([1e-9 * np.abs(ref[i, :3]), [1e-4] * 3])  # period/rms/chi2 relative; dark/angles
+        assert np.all(np.abs(g[i] - ref[i]) <= np.maximum(tol, 2 * spread)), (i, g[i], ref[i], fma[i])
+
+
+def test_checkpoint_resume_reproduces_full_run(tmp_path):
+    """ps_run_files (the process contract): a run resumed from a state file written after line k
+    appends exactly the lines a full run writes (reference state format: the line count)."""
+    from periodsearch_amd import run_files
+    text, _ = synth.generate(synth.config0(n_dense=4, dense_points=30, stop_condition=5, per_end=7.3))
+    inp = tmp_path / "period_search_in"
+    inp.write_text(text)
+    full, st_full = tmp_path / "out_full", tmp_path / "state_full"
+    run_files(str(inp), str(full), str(st_full))
+    lines = full.read_text().splitlines(keepends=True)
+    assert len(lines) > 10 and st_full.read_text().strip() == str(len(lines))
+    k = len(lines) // 2
+    part, st_part = tmp_path / "out_part", tmp_path / "state_part"
+    part.write_text("".join(lines[:k]))
+    st_part.write_text(str(k))
+    run_files(str(inp), str(part), str(st_part))
+    assert part.read_text() == full.read_text()
+    assert st_part.read_text().strip() == str(len(lines))
