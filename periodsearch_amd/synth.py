@@ -138,7 +138,10 @@ def generate(spec: SynthSpec):
             y = y * (1 + spec.dense_noise * rng.standard_normal(len(y)))
             lcs.append((0, jd, y / y.mean(), sun, earth))
     if spec.n_sparse:
-        jd = np.sort(rng.choice(ok, size=spec.n_sparse, replace=False) + rng.uniform(0, 0.5, spec.n_sparse))
+        # one point per observable night while there are enough nights, else several per night (sparse
+        # surveys revisit a field 2-3 times a night)
+        jd = np.sort(rng.choice(ok, size=spec.n_sparse, replace=spec.n_sparse > len(ok)) +
+                     rng.uniform(0, 0.5, spec.n_sparse))
         sun, earth = _positions(jd)
         y = model_brightness(nor, darea, dsph, coef, lam, beta, spec.period_h, jd, sun, earth, spec.phase, spec.cl)
         y = y * (1 + spec.sparse_noise * rng.standard_normal(len(y)))

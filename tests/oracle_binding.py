@@ -110,12 +110,12 @@ class OracleCtx:
         return cls(p, fma)
 
     @classmethod
-    def from_text(cls, text):
+    def from_text(cls, text, fma=False):
         err = C.c_int()
-        p = oracle().pso_ctx_from_text(text.encode(), C.byref(err))
+        p = oracle(fma).pso_ctx_from_text(text.encode(), C.byref(err))
         if not p:
             raise RuntimeError(f"oracle cannot parse input: {err.value}")
-        return cls(p)
+        return cls(p, fma)
 
     def __del__(self):
         try:
