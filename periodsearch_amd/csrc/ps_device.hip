@@ -190,7 +190,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     const int npts = h->npts, nobs = in->ndata, nf = geo->nfac;
     // points (SoA); the convexity rows have y = 0, weight 1, t/e/e0 unused (:455-456, :721-726)
     std::vector<double> t(npts, 0), ex(npts, 0), ey(npts, 0), ez(npts, 0), sx(npts, 0), sy(npts, 0), sz(npts, 0),
-        y(npts, 0), w(npts, 1), sig(npts, 1);
+        alpha(npts, 0), y(npts, 0), w(npts, 1), sig(npts, 1);
     h->s2i_host.assign(npts, 0);
     for (int p = 0; p < nobs; ++p) {
         t[p] = in->tim[p];
@@ -200,6 +200,9 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
         sx[p] = in->ee0[3 * p];
         sy[p] = in->ee0[3 * p + 1];
         sz[p] = in->ee0[3 * p + 2];
+        // phase angle: a constant of the point, from the ecliptic-frame directions (bright.c:30-31,
+        // dot_product.c:14), with the host libm's acos
+        alpha[p] = std::acos(ex[p] * sx[p] + ey[p] * sy[p] + ez[p] * sz[p]);
         y[p] = in->brightness[p];
         w[p] = in->weight[p];
         sig[p] = in->sig[p];
@@ -292,6 +295,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.sx = dev_upload(sx.data(), npts, o);
     P.sy = dev_upload(sy.data(), npts, o);
     P.sz = dev_upload(sz.data(), npts, o);
+    P.alpha = dev_upload(alpha.data(), npts, o);
     P.y = dev_upload(y.data(), npts, o);
     P.w = dev_upload(w.data(), npts, o);
     P.sig = dev_upload(sig.data(), npts, o);

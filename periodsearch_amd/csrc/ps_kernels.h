@@ -52,6 +52,7 @@ struct DevProblem {
     double phi0, alamda_start, alamda_incr;
     double par1, par2, par3, log_cl;
     const double *t, *ex, *ey, *ez, *sx, *sy, *sz, *y, *s2i, *w, *sig;
+    const double *alpha;  // phase angle of each point, acos(ee . ee0)
     const int *lc_start, *lc_inrel, *lc_of;
     // phase B tiles (16 observed points each) split into runs of one lightcurve ("segments"):
     // tile t has segments seg_first[t] .. seg_first[t+1]-1, segment s covers tile-relative points
@@ -227,16 +228,24 @@ __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, c
 }
 
 // Phase function of a point and its derivatives w.r.t. the three phase parameters (phasec.c:11-24,
-// bright.c:30-35).  The phase angle comes from the body-frame directions: cos alpha = e.e0 is
-// invariant under the common rotation.
-__device__ inline double phase_scale(const double *a, int nc, const double e[3], const double e0[3], double dphp[3]) {
-    const double alpha = acos(e[0] * e0[0] + e[1] * e0[1] + e[2] * e0[2]);
-    const double p1 = a[nc + 3], p2 = a[nc + 4], p3 = a[nc + 5];
-    const double ephi = exp(-alpha / p2);
+// bright.c:33-35).  The phase angle alpha = acos(ee . ee0) (bright.c:30-31) depends on the point only
+// and is computed once per problem on the host.  The two terms that need exp and a division,
+// exp(-alpha/p2) and p1*exp(-alpha/p2)*alpha/p2^2, are formed once per evaluation by phase_terms; the
+// facet-loop epilogue only combines them.
+__device__ inline void phase_terms(const DevProblem &P, const double *a, double *ephi, double *dph1, size_t pitch) {
+    const double p1 = a[P.nc + 3], p2 = a[P.nc + 4];
+    for (int p = threadIdx.x; p < P.nobs; p += kWG) {
+        const double alpha = P.alpha[p];
+        const double e = exp(-alpha / p2);
+        ephi[p] = e;
+        dph1[size_t(p) * pitch] = p1 * e * alpha / (p2 * p2);
+    }
+}
+__device__ inline double phase_scale(const double *a, int nc, double alpha, double ephi, double dph1, double dphp[3]) {
     dphp[0] = ephi;
-    dphp[1] = p1 * ephi * alpha / (p2 * p2);
+    dphp[1] = dph1;
     dphp[2] = alpha;
-    return 1 + p1 * ephi + p3 * alpha;
+    return 1 + a[nc + 3] * ephi + a[nc + 5] * alpha;
 }
 
 // Brightness derivatives w.r.t. beta, lambda, omega before the phase factor (bright.c:88-104 with
@@ -441,9 +450,8 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
         double scale[kTP], dphp[kTP][3];
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
-            double v[3], v0[3];
-            dir(h, v, v0);
-            scale[h] = phase_scale(a, nc, v, v0, dphp[h]);
+            const int p = valid[h] ? (kTP * tp + h) * 16 + pl : 0;
+            scale[h] = phase_scale(a, nc, P.alpha[p], ytemp[p], J[size_t(p) * kMap + kRes], dphp[h]);
         }
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
@@ -569,6 +577,9 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     const int ma = P.ma, npts = P.npts, mfit = P.mfit;
     curv_lds(P, sh.a, sh);
     blmatrix_lds(sh.a[P.nc], sh.a[P.nc + 1], sh);
+    // exp(-alpha/p2) into ytemp (phase B replaces it by the model brightness of the point) and the p2
+    // derivative of the phase function into the residual column of J (free until phase D)
+    phase_terms(P, sh.a, ytemp, J + kRes, kMap);
     if (LDSB) {
         if constexpr (MIX)
             for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = P.dsphf[i];
