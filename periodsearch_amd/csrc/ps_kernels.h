@@ -91,6 +91,7 @@ struct Shared {
     int ipiv[64];
     double scal[8];
     int iscal[4];
+    int next_tp;            // phase B: the next unclaimed tile pair
     double bl[9], dbl[18];  // pole rotation and its beta/lambda derivatives (blmatrix)
     double hbl[18];         // dbl_j * bl^T: the beta/lambda derivative generators in the pole frame
     union {
@@ -348,6 +349,15 @@ struct TileSums {
     double br;
 };
 
+// The next tile pair of phase B for a wave, first come first served.  The two waves that share a SIMD
+// do not progress at the same rate (the older one issues first); with a static split the slower one
+// finishes its last tiles alone, at well under half the pair's combined rate.
+__device__ inline int next_tile_pair(Shared &sh, int lane) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(&sh.next_tp, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 template <int NT, int R, bool LDSB, bool MIX>
 __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp, double *part) {
     using Pr = Prec<MIX>;
@@ -368,7 +378,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
     const double cl_d = exp(a[nc + 6]);
     const int ntiles = (P.nobs + 15) >> 4;
     const int nchunk = P.nf >> 2;  // nf = 8 nrows^2
-    for (int tp = wave; kTP * tp < ntiles; tp += kWaves) {
+    for (int tp = wave; kTP * tp < ntiles; tp = next_tile_pair(sh, lane)) {
         T e[kTP][3], e0[kTP][3];
         double rot[kTP][3];                 // sin phi, cos phi, t: all the epilogue needs of the point
         double ed[kTP][MIX ? 3 : 1], e0d[kTP][MIX ? 3 : 1];  // FP64 directions when e, e0 are FP32
@@ -541,17 +551,42 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
             }
         }
     }
-    // convexity rows: res_c = sum Area*n_c, dres_k = sum Area*Dsph_k*n_c (conv.c; Area = Darea*g), FP64
-    for (int it = threadIdx.x; it < 3 * (nc + 1); it += kWG) {
-        const int c = it / (nc + 1), k = it - c * (nc + 1);
-        const double *nn = (c == 0) ? sh.nx : (c == 1) ? sh.ny : sh.nz;
-        double s = 0;
-        if (k == nc) {
-            for (int i = 0; i < P.nf; ++i) s += sh.area[i] * nn[i];
-            ytemp[P.nobs + c] = s;
-        } else {
-            for (int i = 0; i < P.nf; ++i) s += sh.area[i] * P.dsph[i * pitch + k] * nn[i];
-            J[size_t(P.nobs + c) * kMap + k] = s;
+    // convexity rows: res_c = sum Area*n_c, dres_k = sum Area*Dsph_k*n_c (conv.c:12-29; Area = Darea*g),
+    // FP64, on the last four waves: lane (k, q) sums the three components for coefficient k (k == nc:
+    // the plain sums of Area*n) over facets q, q+4, ..., and the four quarters are added across the lane
+    // quad.  Dsph comes from LDS where its FP64 copy is resident.
+    constexpr int kConvWaves = 4;  // 16 coefficients per wave: nc <= 63
+    if (wave >= kWaves - kConvWaves) {
+        const double *cds;
+        if constexpr (LDSB && !MIX)
+            cds = sh.u.dsph;
+        else
+            cds = P.dsph;
+        const int q = lane & 3, k = (wave - (kWaves - kConvWaves)) * 16 + (lane >> 2);
+        double s0 = 0, s1 = 0, s2 = 0;
+        if (k <= nc) {
+#pragma unroll 4
+            for (int i = q; i < P.nf; i += 4) {
+                const double t = sh.area[i] * (k < nc ? cds[i * pitch + k] : 1.0);
+                s0 += t * sh.nx[i];
+                s1 += t * sh.ny[i];
+                s2 += t * sh.nz[i];
+            }
+        }
+        s0 += dpp<0xB1>(s0);  // lane ^ 1
+        s1 += dpp<0xB1>(s1);
+        s2 += dpp<0xB1>(s2);
+        s0 += dpp<0x4E>(s0);  // lane ^ 2
+        s1 += dpp<0x4E>(s1);
+        s2 += dpp<0x4E>(s2);
+        if (q == 0 && k < nc) {
+            J[size_t(P.nobs) * kMap + k] = s0;
+            J[size_t(P.nobs + 1) * kMap + k] = s1;
+            J[size_t(P.nobs + 2) * kMap + k] = s2;
+        } else if (q == 0 && k == nc) {
+            ytemp[P.nobs] = s0;
+            ytemp[P.nobs + 1] = s1;
+            ytemp[P.nobs + 2] = s2;
         }
     }
     for (int it = threadIdx.x; it < 3 * (P.ma - nc); it += kWG) {
@@ -580,6 +615,7 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     // exp(-alpha/p2) into ytemp (phase B replaces it by the model brightness of the point) and the p2
     // derivative of the phase function into the residual column of J (free until phase D)
     phase_terms(P, sh.a, ytemp, J + kRes, kMap);
+    if (tid == 0) sh.next_tp = kWaves;  // tile pairs 0..7 go to waves 0..7
     if (LDSB) {
         if constexpr (MIX)
             for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = P.dsphf[i];
