@@ -60,6 +60,7 @@ Variant pick_variant(int nc, int nf) {
 struct ps_handle {
     int device = 0;
     int precision = PS_FP64;
+    bool ls = false;  // the fit needs the Lambert column (cl fitted)
     int ndata = 0, npts = 0, nlc = 0, nfac = 0, ncoef = 0, ma = 0, mfit = 0, nmax = 0, slots = 0;
     double freq_start = 0, freq_end = 0, freq_step = 0, escl = 0, conw_r0 = 0;
     std::vector<double> s2i_host;  // 1/sig^2 per point (convexity rows set per call)
@@ -88,9 +89,9 @@ namespace {
         if (e_ != hipSuccess) return ps_fail(PS_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_));          \
     } while (0)
 
-template <int NT, int R, bool LDSB, bool MIX>
+template <int NT, int R, bool LDSB, bool MIX, bool LS>
 int launch_fit(ps_handle *h, int grid, int nfits, double conw_r) {
-    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB, MIX>), dim3(grid), dim3(kWG), 0, h->stream, h->dP, h->S, h->d_freq,
+    hipLaunchKernelGGL((psk::fit_kernel<NT, R, LDSB, MIX, LS>), dim3(grid), dim3(kWG), 0, h->stream, h->dP, h->S, h->d_freq,
                        h->d_pole, nfits, conw_r, h->d_out);
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
@@ -266,6 +267,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.npairs = geo->npairs;
     P.mmax1 = geo->mmax + 1;
     P.ia_cl = ia[ncoef + 6];
+    h->ls = P.ia_cl != 0;
     P.ia_ls = ia[ncoef + 7];
     // LM stop rule (:847-860)
     P.n_iter_max = 0;
@@ -339,7 +341,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(ps_fail(PS_ERR_DEVICE, "device query failed"));
     int per_cu = 1;
     dispatch(h->var, false, [&]<int NT, int R, bool LDSB, bool MIX>() {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::fit_kernel<NT, R, LDSB, MIX>, kWG, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::fit_kernel<NT, R, LDSB, MIX, false>, kWG, 0);
         return 0;
     });
     h->slots = std::max(1, per_cu) * prop.multiProcessorCount;
@@ -422,7 +424,8 @@ extern "C" int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const i
     const int grid = std::min(nfits, h->slots);
     HIPCHECK(hipEventRecord(h->ev0, h->stream));
     rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
-        return launch_fit<NT, R, LDSB, MIX>(h, grid, nfits, conw_r);
+        return h->ls ? launch_fit<NT, R, LDSB, MIX, true>(h, grid, nfits, conw_r)
+                     : launch_fit<NT, R, LDSB, MIX, false>(h, grid, nfits, conw_r);
     });
     if (rc != PS_OK) return ps_fail(rc, "fit kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIPCHECK(hipEventRecord(h->ev1, h->stream));

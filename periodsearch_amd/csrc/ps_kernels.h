@@ -337,7 +337,9 @@ __device__ inline double sum_row16(double x) {  // sum over the lane's 16-lane r
 //   X   = A*s            = Ac*mm + As*mm/(mu+mu0),       mm = vis*mu*mu0
 //   ad  = A*ds/dmu       = vis*(As*(mu0/(mu+mu0))^2 + Ac*mu0),  ad0 likewise with mu
 // (bright.c:69-76).  MIX: facet terms, rotation sums and the shape contraction in FP32; the
-// body-frame directions and the brightness accumulation stay FP64.
+// body-frame directions and the brightness accumulation stay FP64.  LS: also the Lambert and
+// Lommel-Seeliger columns, which only a fit of cl (the Lommel-Seeliger weight is always fixed,
+// period_search_BOINC.cpp:570-572) and the raw rows of ps_eval need.
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool MIX>
 struct TileSums {
@@ -358,7 +360,7 @@ __device__ inline int next_tile_pair(Shared &sh, int lane) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
-template <int NT, int R, bool LDSB, bool MIX>
+template <int NT, int R, bool LDSB, bool MIX, bool LS>
 __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, double *J, double *ytemp, double *part) {
     using Pr = Prec<MIX>;
     using T = typename Pr::T;
@@ -410,7 +412,8 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
         }
         for (int c = 0; c < nchunk; ++c) {
             const int j = 4 * c + kg;
-            const T n0 = snx[j], n1 = sny[j], n2 = snz[j], A = sar[j], Ac = sac[j], As = sas[j];
+            const T n0 = snx[j], n1 = sny[j], n2 = snz[j], Ac = sac[j], As = sas[j];
+            const T A = LS ? sar[j] : T(0);
             const T *brow = dsph + j * pitch;
             T B[NT], Br[R > 0 ? R : 1];
 #pragma unroll
@@ -424,15 +427,18 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                 const T mu0 = e0[h][0] * n0 + e0[h][1] * n1 + e0[h][2] * n2;
                 const T vis = (valid[h] && mu > T(kTiny) && mu0 > T(kTiny)) ? T(1) : T(0);
                 const T rd = Pr::rcp(fmax(mu + mu0, T(kTiny)));
-                const T mm = vis * (mu * mu0);
-                const T mr = mm * rd;
-                const T X = Ac * mm + As * mr;
+                const T muv = vis * mu, mu0v = vis * mu0;
+                const T mm = muv * mu0;              // mu*mu0 of a visible facet, else 0
+                const T X = mm * fma(As, rd, Ac);    // Area*s = Area*mu*mu0*(cl + cls/(mu+mu0))
                 s.br += double(X);
-                s.d1 += A * mm;
-                s.d2 += A * mr;
-                const T q0 = mu0 * rd, q = mu * rd;
-                const T ad = vis * (As * (q0 * q0) + Ac * mu0);
-                const T ad0 = vis * (As * (q * q) + Ac * mu);
+                if constexpr (LS) {
+                    s.d1 += A * mm;
+                    s.d2 += A * (mm * rd);
+                }
+                // Area*dsmu = Area*(cls*(mu0/dnom)^2 + cl*mu0) = mu0*(As*mu0/dnom^2 + Ac), dsmu0 likewise
+                const T h2 = As * (rd * rd);
+                const T ad = mu0v * fma(h2, mu0, Ac);
+                const T ad0 = muv * fma(h2, mu, Ac);
                 s.S[0] += ad * n0; s.S[1] += ad * n1; s.S[2] += ad * n2;
                 s.U[0] += ad0 * n0; s.U[1] += ad0 * n1; s.U[2] += ad0 * n2;
 #pragma unroll
@@ -509,7 +515,11 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                 S[k] = double(sum_groups(s.S[k]));
                 U[k] = double(sum_groups(s.U[k]));
             }
-            const double d1 = double(sum_groups(s.d1)), d2 = double(sum_groups(s.d2));
+            double d1 = 0, d2 = 0;  // the Lambert and Lommel-Seeliger columns, when asked for
+            if constexpr (LS) {
+                d1 = double(sum_groups(s.d1));
+                d2 = double(sum_groups(s.d2));
+            }
             T racc[R > 0 ? R : 1];
 #pragma unroll
             for (int r = 0; r < R; ++r) racc[r] = sum_groups(s.racc[r]);
@@ -526,8 +536,8 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                 for (int k = 0; k < 3; ++k) ov[R + k] = scale[h] * js[k];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) ov[R + 3 + i] = br * dphp[h][i];
-                ov[R + 6] = scale[h] * d1 * cl_d;
-                ov[R + 7] = scale[h] * d2;
+                ov[R + 6] = LS ? scale[h] * d1 * cl_d : 0.0;
+                ov[R + 7] = LS ? scale[h] * d2 : 0.0;
                 ov[R + 8] = br * scale[h];
             }
             auto ocol = [&](int k) { return (k < R) ? 16 * NT + k : (k < R + 8) ? nc + (k - R) : P.ma; };
@@ -603,7 +613,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
 // those rows.  (Accumulated in FP32 it is not, and on these systems -- condition numbers 1e8-1e10
 // after damping -- the LM steps along the ill-conditioned directions become noise.)
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB, bool MIX>
+template <int NT, int R, bool LDSB, bool MIX, bool LS>
 __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double *ytemp, double *dave, double *alpha, double *beta,
                          Shared &sh, double *raw_y, double *raw_dyda) {
     using Pd = Prec<false>;  // phase D arithmetic
@@ -629,7 +639,7 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
             sh.aclsf[i] = float(sh.acls[i]);
         }
     __syncthreads();
-    model_rows<NT, R, LDSB, MIX>(P, sh, J, ytemp, dave);
+    model_rows<NT, R, LDSB, MIX, LS>(P, sh, J, ytemp, dave);
     __syncthreads();
     if (raw_y) {  // raw bright()/conv() rows, before the lightcurve normalisation below
         for (int p = tid; p < npts; p += kWG) raw_y[p] = ytemp[p];
@@ -965,7 +975,7 @@ __device__ inline void load_facets(const DevProblem &P, Shared &sh) {
 // ---------------------------------------------------------------------------------------------
 // One (period, pole) fit per workgroup iteration: period_search_BOINC.cpp:819-911 with mrqmin.c.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int R, bool LDSB, bool MIX>
+template <int NT, int R, bool LDSB, bool MIX, bool LS>
 __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__ Pp, Slots S, const double *freq,
                                                   const int *pole, int nfits, double conw_r, ps_fit_result *out) {
     // The problem description is read through a pointer (scalar loads on demand) rather than passed
@@ -1046,7 +1056,7 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
                 __syncthreads();
                 // the starting point fills the current buffers, a trial point the other ones
                 const bool other = !init;
-                const double chi = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0,
+                const double chi = mrqcof<NT, R, LDSB, MIX, LS>(P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0,
                                                             (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr);
                 if (init) {
                     ochisq = chi;
@@ -1131,7 +1141,7 @@ __global__ __launch_bounds__(kWG) void eval_kernel(const DevProblem *__restrict_
     for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
         if (tid < P.ma) sh.a[tid] = avec[size_t(v) * P.ma + tid];
         __syncthreads();
-        const double c = mrqcof<NT, R, LDSB, MIX>(P, J, ytemp, dave, al, be, sh,
+        const double c = mrqcof<NT, R, LDSB, MIX, true>(P, J, ytemp, dave, al, be, sh,
                                              raw_y ? raw_y + size_t(v) * P.npts : nullptr,
                                              raw_dyda ? raw_dyda + size_t(v) * P.npts * P.ma : nullptr);
         for (int it = tid; it < mfit * mfit; it += kWG)
