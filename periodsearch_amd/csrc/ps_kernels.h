@@ -136,6 +136,14 @@ struct Prec<true> {
     __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
+// A load from a pointer held in the problem description, marked as global memory.  Those pointers
+// are read from memory, so the compiler cannot infer their address space and would issue flat loads,
+// which count against the LDS wait counter too: an LDS read after them waits for the global data.
+template <class T>
+__device__ inline T gload(const T *p, size_t i) {
+    return reinterpret_cast<const __attribute__((address_space(1))) T *>(reinterpret_cast<unsigned long long>(p))[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // curv (period_search/curv.c:10-39): g = exp(sum_{m,l} Pleg*(c*Fc + s*Fs)), Area = Darea*g, and the
 // facet loop's Area*cl and Area*cls.
@@ -148,16 +156,16 @@ __device__ inline void curv_lds(const DevProblem &P, const double *a, Shared &sh
         const double *pl = P.pleg + size_t(i) * P.npairs;
         const double *fc = P.fc + size_t(i) * P.mmax1, *fs = P.fs + size_t(i) * P.mmax1;
         for (int q = 0; q < P.npairs; ++q) {
-            const int m = P.pair_m[q];
-            double fsum = a[n] * fc[m];
+            const int m = gload(P.pair_m, q);
+            double fsum = a[n] * gload(fc, m);
             ++n;
             if (m != 0) {
-                fsum = fsum + a[n] * fs[m];
+                fsum = fsum + a[n] * gload(fs, m);
                 ++n;
             }
-            gsum = gsum + pl[q] * fsum;
+            gsum = gsum + gload(pl, q) * fsum;
         }
-        const double ar = P.darea[i] * exp(gsum);
+        const double ar = gload(P.darea, i) * exp(gsum);
         sh.area[i] = ar;
         sh.acl[i] = ar * cl;
         sh.acls[i] = ar * cls;
@@ -204,10 +212,10 @@ __device__ inline void rotation_phase(double omg, double t, double phi0, double 
 // Earth and Sun directions of point p in the body frame (bright.c:41-57 with matrix.c).
 __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, const Shared &sh, double e[3],
                                   double e0[3], double rot[3]) {
-    const double ee[3] = {P.ex[p], P.ey[p], P.ez[p]};
-    const double ee0[3] = {P.sx[p], P.sy[p], P.sz[p]};
+    const double ee[3] = {gload(P.ex, p), gload(P.ey, p), gload(P.ez, p)};
+    const double ee0[3] = {gload(P.sx, p), gload(P.sy, p), gload(P.sz, p)};
     double sf, cf;
-    rot[2] = P.t[p];
+    rot[2] = gload(P.t, p);
     rotation_phase(a[P.nc + 2], rot[2], P.phi0, sf, cf);
     rot[0] = sf;
     rot[1] = cf;
@@ -236,7 +244,7 @@ __device__ inline void point_dirs(const DevProblem &P, const double *a, int p, c
 __device__ inline void phase_terms(const DevProblem &P, const double *a, double *ephi, double *dph1, size_t pitch) {
     const double p1 = a[P.nc + 3], p2 = a[P.nc + 4];
     for (int p = threadIdx.x; p < P.nobs; p += kWG) {
-        const double alpha = P.alpha[p];
+        const double alpha = gload(P.alpha, p);
         const double e = exp(-alpha / p2);
         ephi[p] = e;
         dph1[size_t(p) * pitch] = p1 * e * alpha / (p2 * p2);
@@ -414,12 +422,11 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
             const int j = 4 * c + kg;
             const T n0 = snx[j], n1 = sny[j], n2 = snz[j], Ac = sac[j], As = sas[j];
             const T A = LS ? sar[j] : T(0);
-            const T *brow = dsph + j * pitch;
             T B[NT], Br[R > 0 ? R : 1];
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) B[tt] = brow[16 * tt + pl];
+            for (int tt = 0; tt < NT; ++tt) B[tt] = LDSB ? dsph[j * pitch + 16 * tt + pl] : gload(dsph, j * pitch + 16 * tt + pl);
 #pragma unroll
-            for (int r = 0; r < R; ++r) Br[r] = brow[16 * NT + r];
+            for (int r = 0; r < R; ++r) Br[r] = LDSB ? dsph[j * pitch + 16 * NT + r] : gload(dsph, j * pitch + 16 * NT + r);
 #pragma unroll
             for (int h = 0; h < kTP; ++h) {
                 TileSums<NT, R, MIX> &s = ts[h];
@@ -467,7 +474,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
             const int p = valid[h] ? (kTP * tp + h) * 16 + pl : 0;
-            scale[h] = phase_scale(a, nc, P.alpha[p], ytemp[p], J[size_t(p) * kMap + kRes], dphp[h]);
+            scale[h] = phase_scale(a, nc, gload(P.alpha, p), gload(ytemp, p), gload(J, size_t(p) * kMap + kRes), dphp[h]);
         }
 #pragma unroll
         for (int h = 0; h < kTP; ++h) {
@@ -577,7 +584,8 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
         if (k <= nc) {
 #pragma unroll 4
             for (int i = q; i < P.nf; i += 4) {
-                const double t = sh.area[i] * (k < nc ? cds[i * pitch + k] : 1.0);
+                const double ds = (LDSB && !MIX) ? cds[i * pitch + k] : gload(cds, i * pitch + k);
+                const double t = sh.area[i] * (k < nc ? ds : 1.0);
                 s0 += t * sh.nx[i];
                 s1 += t * sh.ny[i];
                 s2 += t * sh.nz[i];
@@ -628,9 +636,9 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     if (tid == 0) sh.next_tp = kWaves;  // tile pairs 0..7 go to waves 0..7
     if (LDSB) {
         if constexpr (MIX)
-            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = P.dsphf[i];
+            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = gload(P.dsphf, i);
         else
-            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = P.dsph[i];
+            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = gload(P.dsph, i);
     }
     if constexpr (MIX)
         for (int i = tid; i < P.nf; i += kWG) {
@@ -703,18 +711,22 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     struct RowV {
         double v[4], wq;
     };
+    // loads are unconditional (unused columns and rows past the end read column 0 / row 0) so that no
+    // branch and no wait separates them from the LDS reads of the normalisation
+    int coll[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) coll[u] = (u < nb && colu[u] >= 0) ? colu[u] : 0;
     auto stage1 = [&](int s, RowG &g) {
         const int p = 4 * s + kg;
         g.in = s < nsteps && p < npts;
-        const int pp = g.in ? p : 0;
-        const double *jr = J + size_t(pp) * kMap;
+        const size_t pp = g.in ? p : 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) g.raw[u] = (u < nb && colu[u] >= 0 && colu[u] != kRes) ? jr[colu[u]] : 0.0;
-        g.wq = g.in ? P.s2i[pp] * P.w[pp] : 0.0;
-        g.yt = ytemp[pp];
-        g.sig = P.sig[pp];
-        g.y = P.y[pp];
-        g.L = P.lc_of[pp];
+        for (int u = 0; u < 4; ++u) g.raw[u] = gload(J, pp * kMap + coll[u]);
+        g.wq = gload(P.s2i, pp) * gload(P.w, pp);
+        g.yt = gload(ytemp, pp);
+        g.sig = gload(P.sig, pp);
+        g.y = gload(P.y, pp);
+        g.L = gload(P.lc_of, pp);
     };
     auto stage2 = [&](const RowG &g, RowV &r) {
         const bool rel = rell[g.L];
@@ -729,7 +741,7 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
                                   : (rel ? ((col == 0) ? 0.0 : coef * (g.raw[u] - g.yt * m[col])) : g.raw[u]);
             r.v[u] = g.in ? x : 0.0;
         }
-        r.wq = g.wq;
+        r.wq = g.in ? g.wq : 0.0;
     };
     RowG g1, g2;
     RowV cur;
