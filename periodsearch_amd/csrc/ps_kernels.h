@@ -700,9 +700,8 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
 #pragma unroll
     for (int b = 0; b < 10; ++b) acc[b] = Pd::zero();
     const int nsteps = (npts + 3) >> 2;
-    // Two-stage software pipeline over the K-steps: the global loads of step s + 2 (raw columns and
-    // the point's weight, model value, observation and lightcurve) are in flight while step s + 1 is
-    // normalised from the LDS table and step s runs on the MFMA.
+    // Software pipeline over the K-steps: stage1 issues the global loads of a step (raw columns and the
+    // point's weight, model value, observation and lightcurve), stage2 normalises it from the LDS table.
     struct RowG {
         double raw[4], wq, yt, sig, y;
         int L;
@@ -743,24 +742,37 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         }
         r.wq = g.in ? g.wq : 0.0;
     };
-    RowG g1, g2;
-    RowV cur;
-    stage1(wave, g1);
-    stage1(wave + kWaves, g2);
-    stage2(g1, cur);
-    for (int s = wave; s < nsteps; s += kWaves) {
-        RowG g3;
-        stage1(s + 2 * kWaves, g3);
-        RowV nxt;
-        stage2(g2, nxt);
+    auto accumulate = [&](const RowV &r) {
         int idx = 0;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj <= bi; ++bj, ++idx)  // blocks beyond nb add zeros
-                acc[idx] = Pd::mma(cur.v[bi], cur.wq * cur.v[bj], acc[idx]);
+                acc[idx] = Pd::mma(r.v[bi], r.wq * r.v[bj], acc[idx]);
+    };
+    // Three load buffers, unrolled by three so that none is copied (a register copy of fresh load data
+    // would wait for it): the loads of step t + 3 are issued while step t + 1 is normalised and step t
+    // runs on the MFMA.  Steps past the end load row 0 and contribute zeros.
+    constexpr int K = kWaves;
+    RowG ga, gb, gc;
+    RowV cur, nxt;
+    stage1(wave, ga);
+    stage1(wave + K, gb);
+    stage1(wave + 2 * K, gc);
+    stage2(ga, cur);
+    for (int s = wave; s < nsteps; s += 3 * K) {
+        stage1(s + 3 * K, ga);
+        stage2(gb, nxt);
+        accumulate(cur);
         cur = nxt;
-        g2 = g3;
+        stage1(s + 4 * K, gb);
+        stage2(gc, nxt);
+        accumulate(cur);
+        cur = nxt;
+        stage1(s + 5 * K, gc);
+        stage2(ga, nxt);
+        accumulate(cur);
+        cur = nxt;
     }
     __syncthreads();
     // deterministic cross-wave sum into u.d.red (lower blocks; diagonal blocks in full)
