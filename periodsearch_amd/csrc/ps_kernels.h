@@ -727,18 +727,27 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         g.y = gload(P.y, pp);
         g.L = gload(P.lc_of, pp);
     };
+    // per-lane column kinds, fixed for the evaluation: 0 none, 1 residual, 2 c00 (zero in relative
+    // curves), 3 any other; stage2 reads the table unconditionally (column 0 where unused) and selects,
+    // so it has no branches
+    int kind[4], colt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int col = colu[u];
+        kind[u] = !(u < nb && col >= 0) ? 0 : (col == kRes) ? 1 : (col == 0) ? 2 : 3;
+        colt[u] = kind[u] == 3 ? col : 0;
+    }
     auto stage2 = [&](const RowG &g, RowV &r) {
         const bool rel = rell[g.L];
-        const double coef = rel ? g.sig * cfl[g.L] : 1.0;
+        const double cf = cfl[g.L];
+        const double coef = rel ? g.sig * cf : 1.0;
         const double *m = tab + g.L * tpitch;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int col = colu[u];
-            double x = 0;
-            if (u < nb && col >= 0)
-                x = (col == kRes) ? g.y - coef * g.yt
-                                  : (rel ? ((col == 0) ? 0.0 : coef * (g.raw[u] - g.yt * m[col])) : g.raw[u]);
-            r.v[u] = g.in ? x : 0.0;
+            const double xr = coef * (g.raw[u] - g.yt * m[colt[u]]);
+            const double xo = rel ? ((kind[u] == 2) ? 0.0 : xr) : g.raw[u];
+            const double x = (kind[u] == 1) ? g.y - coef * g.yt : xo;
+            r.v[u] = (g.in && kind[u] != 0) ? x : 0.0;
         }
         r.wq = g.in ? g.wq : 0.0;
     };
