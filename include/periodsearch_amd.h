@@ -14,6 +14,8 @@
  *   ps_conw_search/ps_conw_round  -> CUDAPrecalc, start_CUDA.h:9-10 (period_search_BOINC.cpp:487-715)
  *   ps_scan                       -> CUDAStart, start_CUDA.h:6-7, without the MFILE (the host writes)
  *   ps_fit_batch                  -> the per-(period, pole) body of the scan, period_search_BOINC.cpp:817-921
+ *   ps_fit_multi                  -> the same body for the fits of several asteroids in one batch (the reference
+ *                                    runs one asteroid per process; no counterpart)
  *   ps_eval                       -> mrqcof (period_search/mrqcof.c:16-148) with bright()'s dyda
  *   ps_release                    -> CUDAUnprepare, start_CUDA.h:14
  *   ps_format_line                -> the out.printf of period_search_BOINC.cpp:927-931
@@ -129,13 +131,21 @@ void ps_geometry_free(ps_geometry *geo);
 
 /* ---- device ---- */
 /* Uploads points, lightcurve table, facet tables and the free/fixed mask; allocates per-slot scratch.
-   Returns PS_ERR_UNSUPPORTED when mfit > 63, ma > 72 or nfac > 512 (Lmax <= 6 and nrows <= 8 are
+   Returns PS_ERR_UNSUPPORTED when mfit > 63, ma > 63 or nfac > 512 (Lmax <= 6 and nrows <= 8 are
    supported; the reference allows up to MAX_N_PAR = 200 parameters and MAX_N_FAC = 1000 facets). */
 int ps_prepare(const ps_input *in, const ps_geometry *geo, int device, ps_handle **out);
 int ps_set_precision(ps_handle *h, int precision);
 int ps_get_info(const ps_handle *h, ps_info *info);
 /* Runs nfits independent fits (freq[i] in 1/h, pole[i] in 1..10) with convexity weight conw_r. */
 int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r, ps_fit_result *out);
+/* The fits of several prepared problems (asteroids) on one device as one batch: job j fits frequency
+   freq[j] (1/h) with pole pole[j] (1..10) of problem hs[prob[j]] at convexity weight
+   conw_r[prob[j]], into out[j].  Each problem's jobs run as one launch on its own stream; all
+   launches are issued, largest problem first, before any result is waited for, so problems too small
+   to fill the device run concurrently.  Results equal ps_fit_batch's for the same fits.  Each problem
+   handle may appear once.  The per-job body is that of period_search_BOINC.cpp:817-921. */
+int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, int njobs, const int *prob, const double *freq,
+                 const int *pole, ps_fit_result *out);
 /* One round of the convexity-weight search over the first min(10, nmax) periods; ave_dark = mean
    best-pole dark-facet % (NaN counted as 1.0). */
 int ps_conw_round(ps_handle *h, double conw_r, double *ave_dark);

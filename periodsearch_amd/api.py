@@ -118,6 +118,9 @@ class PeriodSearch:
                 "ps_fit_batch")
         return out
 
+    def _handle(self):
+        return self._need()
+
     def conw_round(self, conw_r):
         ave = C.c_double()
         L.check(self._lib.ps_conw_round(self._need(), float(conw_r), C.byref(ave)), "ps_conw_round")
@@ -193,3 +196,25 @@ def run_files(in_path, out_path, state_path=None, device=0, precision=L.PS_FP64)
     L.check(L.lib().ps_run_files(str(in_path).encode(), str(out_path).encode(),
                                  str(state_path).encode() if state_path else None, device, precision),
             "ps_run_files")
+
+
+def fit_multi(searches, conw_r, prob, freq, pole):
+    """Fits of several prepared problems as one batch (ps_fit_multi): job j fits freq[j] with pole
+    pole[j] of searches[prob[j]] at convexity weight conw_r[prob[j]].  Returns FIT_DTYPE records in job
+    order."""
+    searches = list(searches)
+    if not searches:
+        raise ValueError("fit_multi: no problems")
+    lib = searches[0]._lib
+    ptrs = [s._handle() for s in searches]
+    hs = (C.c_void_p * len(searches))(*[p.value if isinstance(p, C.c_void_p) else p for p in ptrs])
+    cw = np.ascontiguousarray(conw_r, dtype=np.float64)
+    prob = np.ascontiguousarray(prob, dtype=np.int32)
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    pole = np.ascontiguousarray(pole, dtype=np.int32)
+    if not (len(cw) == len(searches) and len(prob) == len(freq) == len(pole)):
+        raise ValueError("fit_multi: inconsistent sizes")
+    out = np.zeros(len(freq), dtype=FIT_DTYPE)
+    L.check(lib.ps_fit_multi(hs, len(searches), _dp(cw), len(freq), prob.ctypes.data_as(L.IP), _dp(freq),
+                             pole.ctypes.data_as(L.IP), out.ctypes.data_as(C.POINTER(L.PsFitResult))), "ps_fit_multi")
+    return out

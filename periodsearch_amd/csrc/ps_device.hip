@@ -395,19 +395,14 @@ extern "C" int ps_get_info(const ps_handle *h, ps_info *info) {
     return PS_OK;
 }
 
-extern "C" int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r,
-                            ps_fit_result *out) {
-    if (!h || nfits < 0 || (nfits && (!freq || !pole || !out))) return ps_fail(PS_ERR_ARG, "ps_fit_batch: bad argument");
-    if (nfits == 0) return PS_OK;
-    for (int i = 0; i < nfits; ++i)
-        if (pole[i] < 1 || pole[i] > PS_N_POLES || !(freq[i] > 0))
-            return ps_fail(PS_ERR_ARG, "ps_fit_batch: fit %d has pole %d / freq %g", i, pole[i], freq[i]);
-    if (!(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_fit_batch: conw_r must be positive");
-    HIPCHECK(hipSetDevice(h->device));
+namespace {
+// Uploads one batch of fits of a problem and launches the fit kernel on the problem's stream, without
+// waiting: the results land in h->d_out.  Validation is the caller's.
+int enqueue_fits(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r) {
     if (nfits > h->cap_fits) {
-        hipFree(h->d_freq);
-        hipFree(h->d_pole);
-        hipFree(h->d_out);
+        (void)hipFree(h->d_freq);
+        (void)hipFree(h->d_pole);
+        (void)hipFree(h->d_out);
         h->d_freq = nullptr;
         h->d_pole = nullptr;
         h->d_out = nullptr;
@@ -429,12 +424,85 @@ extern "C" int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const i
     });
     if (rc != PS_OK) return ps_fail(rc, "fit kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIPCHECK(hipEventRecord(h->ev1, h->stream));
+    return PS_OK;
+}
+
+// Waits for the batch enqueued last on h and copies its nfits results to out.
+int collect_fits(ps_handle *h, int nfits, ps_fit_result *out) {
     HIPCHECK(hipMemcpyAsync(out, h->d_out, sizeof(ps_fit_result) * nfits, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->kernel_ms += ms;
     h->launches += 1;
+    return PS_OK;
+}
+
+int check_fits(const char *fn, int nfits, const double *freq, const int *pole) {
+    for (int i = 0; i < nfits; ++i)
+        if (pole[i] < 1 || pole[i] > PS_N_POLES || !(freq[i] > 0))
+            return ps_fail(PS_ERR_ARG, "%s: fit %d has pole %d / freq %g", fn, i, pole[i], freq[i]);
+    return PS_OK;
+}
+}  // namespace
+
+extern "C" int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r,
+                            ps_fit_result *out) {
+    if (!h || nfits < 0 || (nfits && (!freq || !pole || !out))) return ps_fail(PS_ERR_ARG, "ps_fit_batch: bad argument");
+    if (nfits == 0) return PS_OK;
+    int rc = check_fits("ps_fit_batch", nfits, freq, pole);
+    if (rc != PS_OK) return rc;
+    if (!(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_fit_batch: conw_r must be positive");
+    HIPCHECK(hipSetDevice(h->device));
+    rc = enqueue_fits(h, nfits, freq, pole, conw_r);
+    return rc != PS_OK ? rc : collect_fits(h, nfits, out);
+}
+
+extern "C" int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, int njobs, const int *prob,
+                            const double *freq, const int *pole, ps_fit_result *out) {
+    if (!hs || nh < 1 || !conw_r || njobs < 0 || (njobs && (!prob || !freq || !pole || !out)))
+        return ps_fail(PS_ERR_ARG, "ps_fit_multi: bad argument");
+    for (int i = 0; i < nh; ++i) {
+        if (!hs[i]) return ps_fail(PS_ERR_ARG, "ps_fit_multi: problem %d is null", i);
+        if (hs[i]->device != hs[0]->device) return ps_fail(PS_ERR_ARG, "ps_fit_multi: problems on different devices");
+        if (!(conw_r[i] > 0)) return ps_fail(PS_ERR_ARG, "ps_fit_multi: conw_r[%d] must be positive", i);
+        for (int k = 0; k < i; ++k)
+            if (hs[k] == hs[i]) return ps_fail(PS_ERR_ARG, "ps_fit_multi: problem %d repeats problem %d", i, k);
+    }
+    for (int j = 0; j < njobs; ++j)
+        if (prob[j] < 0 || prob[j] >= nh) return ps_fail(PS_ERR_ARG, "ps_fit_multi: job %d has problem %d", j, prob[j]);
+    int rc = check_fits("ps_fit_multi", njobs, freq, pole);
+    if (rc != PS_OK) return rc;
+    if (njobs == 0) return PS_OK;
+    HIPCHECK(hipSetDevice(hs[0]->device));
+    // the jobs of each problem, and the problems in order of decreasing work (longest first)
+    std::vector<std::vector<int>> jobs(nh);
+    for (int j = 0; j < njobs; ++j) jobs[prob[j]].push_back(j);
+    std::vector<int> order(nh);
+    for (int i = 0; i < nh; ++i) order[i] = i;
+    auto work = [&](int i) { return double(jobs[i].size()) * hs[i]->npts * hs[i]->nfac * hs[i]->ncoef; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    // every problem's batch goes onto its own stream before any result is waited for, so the launches
+    // of different problems run concurrently on the device
+    std::vector<std::vector<double>> f(nh);
+    std::vector<std::vector<int>> pl(nh);
+    for (int i : order) {
+        if (jobs[i].empty()) continue;
+        for (int j : jobs[i]) {
+            f[i].push_back(freq[j]);
+            pl[i].push_back(pole[j]);
+        }
+        rc = enqueue_fits(hs[i], int(jobs[i].size()), f[i].data(), pl[i].data(), conw_r[i]);
+        if (rc != PS_OK) return rc;
+    }
+    std::vector<ps_fit_result> res;
+    for (int i : order) {
+        if (jobs[i].empty()) continue;
+        res.resize(jobs[i].size());
+        rc = collect_fits(hs[i], int(jobs[i].size()), res.data());
+        if (rc != PS_OK) return rc;
+        for (size_t k = 0; k < jobs[i].size(); ++k) out[jobs[i][k]] = res[k];
+    }
     return PS_OK;
 }
 

@@ -409,3 +409,49 @@ def test_checkpoint_resume_reproduces_full_run(tmp_path):
     run_files(str(inp), str(part), str(st_part))
     assert part.read_text() == full.read_text()
     assert st_part.read_text().strip() == str(len(lines))
+
+
+# ---- several problems in one batch (ps_fit_multi): one launch per problem, all in flight together
+def test_fit_multi_matches_single_problem_fits():
+    """Five problems of four kernel shapes (Lmax 3 twice, Lmax 4, Lmax 3 on 512 facets, the production
+    Lmax 6 shape), 2-3 trial periods x 10 poles each, submitted as one shuffled job list: every result
+    equals that problem's own ps_fit_batch bit for bit.  The Lmax 4 fits (an MFMA tile plus nine VALU
+    coefficient columns) also match the oracle to 1e-9."""
+    from periodsearch_amd import fit_multi
+    specs = [synth.config0(n_dense=4, dense_points=30, stop_condition=10),
+             synth.config0(seed=7, fit_lmax=4, n_dense=6, dense_points=40, stop_condition=10),
+             synth.config0(seed=9, n_dense=5, dense_points=40, stop_condition=10),
+             synth.config0(seed=11, fit_nrows=8, n_dense=4, dense_points=30, stop_condition=10),
+             synth.config1(stop_condition=10)]
+    texts = [synth.generate(s)[0] for s in specs]
+    pss = [PeriodSearch.from_text(t).prepare(device=0) for t in texts]
+    try:
+        assert [p.info()["ncoef"] for p in pss] == [16, 25, 16, 16, 49]
+        assert pss[3].info()["nfac"] == 512
+        cws = [p.info()["conw_r0"] for p in pss]
+        prob, n = [], []
+        for i in range(len(pss)):
+            for k in range(3 if i < 4 else 2):
+                prob += [i] * 10
+                n += [1 + 7 * k] * 10
+        prob, n = np.array(prob), np.array(n)
+        pole = np.tile(np.arange(1, 11), len(prob) // 10)
+        perm = np.random.default_rng(4).permutation(len(prob))
+        prob, n, pole = prob[perm], n[perm], pole[perm]
+        freq = np.array([pss[p].freq(np.array([k]))[0] for p, k in zip(prob, n)])
+        got = fit_multi(pss, cws, prob, freq, pole)
+        for i, ps in enumerate(pss):
+            sel = prob == i
+            ref = ps.fit_batch(freq[sel], pole[sel], cws[i])
+            for k in got.dtype.names:
+                assert np.array_equal(got[k][sel], ref[k]), (i, k)
+        orc = ob.OracleCtx.from_text(texts[1])
+        orc.set_conw(cws[1])
+        sel = np.flatnonzero(prob == 1)[:3]
+        for j in sel:
+            r = orc.fit(float(freq[j]), int(pole[j]))
+            assert got["niter"][j] == r["niter"]
+            assert _rel(got["chisq"][j], r["chisq"]) <= 1e-9 and _rel(got["dev"][j], r["dev_new"]) <= 1e-9
+    finally:
+        for p in pss:
+            p.close()
