@@ -86,8 +86,9 @@ struct Shared {
     double da[64], col[64], b0[64];
     double prow[64], srow[64];  // Gauss-Jordan: the new pivot row, the row moved by a swap
     int indxr[64], indxc[64];
-    double red_v[kWaves], red_s[kWaves];
+    double red_v[kWaves];   // Gauss-Jordan: each wave's pivot candidate |a| and index
     int red_q[kWaves];
+    double piv_s;           // the pivot value, from the pivot row's owner
     int ipiv[64];
     double scal[8];
     int iscal[4];
@@ -824,13 +825,13 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
 // are written back to sh.u.cov / sh.da.
 // ---------------------------------------------------------------------------------------------
 struct PivotCand {
-    double v, s;  // |a|, a
-    int q;        // row * n + column, -1 = none
+    double v;  // |a|
+    int q;     // row * n + column, -1 = none
 };
 
 __device__ inline void pivot_take(PivotCand &c, double s, int q) {
     const double v = fabs(s);
-    if (v > c.v || (v == c.v && q > c.q)) c = {v, s, q};
+    if (v > c.v || (v == c.v && q > c.q)) c = {v, q};
 }
 
 __device__ inline void pivot_merge(PivotCand &c, const PivotCand &o) {
@@ -839,7 +840,7 @@ __device__ inline void pivot_merge(PivotCand &c, const PivotCand &o) {
 
 template <int CTRL>
 __device__ inline PivotCand pivot_dpp(const PivotCand &c) {
-    return {dpp<CTRL>(c.v), dpp<CTRL>(c.s), int(__builtin_amdgcn_mov_dpp(unsigned(c.q), CTRL, 0xF, 0xF, false))};
+    return {dpp<CTRL>(c.v), int(__builtin_amdgcn_mov_dpp(unsigned(c.q), CTRL, 0xF, 0xF, false))};
 }
 
 __device__ inline double swap_half(double x, bool rows32, int k) {  // element k of the permlane swap pair
@@ -861,8 +862,8 @@ __device__ inline PivotCand pivot_wave_max(PivotCand c) {
     for (int r32 = 0; r32 < 2; ++r32) {
         const unsigned q = unsigned(c.q);
         auto qq = r32 ? __builtin_amdgcn_permlane32_swap(q, q, false, false) : __builtin_amdgcn_permlane16_swap(q, q, false, false);
-        PivotCand a{swap_half(c.v, r32, 0), swap_half(c.s, r32, 0), int(qq[0])};
-        const PivotCand b{swap_half(c.v, r32, 1), swap_half(c.s, r32, 1), int(qq[1])};
+        PivotCand a{swap_half(c.v, r32, 0), int(qq[0])};
+        const PivotCand b{swap_half(c.v, r32, 1), int(qq[1])};
         pivot_merge(a, b);
         c = a;
     }
@@ -882,7 +883,7 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform row ownership
     double *a = sh.u.g.cov, *b = sh.da;
     double row[kGJRows];
-    PivotCand c{-1.0, 0.0, -1};
+    PivotCand c{-1.0, -1};
 #pragma unroll
     for (int k = 0; k < kGJRows; ++k) {
         const int ll = wave + kWaves * k;
@@ -895,32 +896,37 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         c = pivot_wave_max(c);
         if (l == 0) {
             sh.red_v[wave] = c.v;
-            sh.red_s[wave] = c.s;
             sh.red_q[wave] = c.q;
         }
         __syncthreads();
-        c = {sh.red_v[0], sh.red_s[0], sh.red_q[0]};
-        for (int w = 1; w < kWaves; ++w) {
-            const PivotCand o{sh.red_v[w], sh.red_s[w], sh.red_q[w]};
-            if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
-        }
+        // the workgroup's candidate: a tree over the eight wave candidates (the order is total, so the
+        // result does not depend on the grouping)
+        PivotCand cw[kWaves];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) cw[w] = {sh.red_v[w], sh.red_q[w]};
+#pragma unroll
+        for (int st = 1; st < kWaves; st *= 2)
+#pragma unroll
+            for (int w = 0; w < kWaves; w += 2 * st) pivot_merge(cw[w], cw[w + st]);
+        c = cw[0];
         if (c.q < 0) return 1;
         const int irow = c.q / n, icol = c.q - irow * n;
-        if (c.s == 0.0) return 2;
         if (tid == 0) {
             sh.indxr[i] = irow;
             sh.indxc[i] = icol;
         }
-        const double pivinv = 1.0 / c.s;
         done |= 1ull << icol;
-        // publish the new pivot row (old row irow scaled, gauss_errc.c:44-53) and, for a swap, old row
-        // icol, which moves to row irow (:38-42)
+        // publish the new pivot row (old row irow scaled, gauss_errc.c:44-53) with the pivot value and,
+        // for a swap, old row icol, which moves to row irow (:38-42)
         if (wave == irow % kWaves) {
             double v = 0;
 #pragma unroll
             for (int k = 0; k < kGJRows; ++k)
                 if (k == irow / kWaves) v = row[k];
+            const double s = read_lane(v, icol);
+            const double pivinv = 1.0 / s;
             if (l <= n) sh.prow[l] = ((l == icol) ? 1.0 : v) * pivinv;
+            if (l == 0) sh.piv_s = s;
         }
         if (irow != icol && wave == icol % kWaves) {
             double v = 0;
@@ -930,10 +936,11 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
             if (l <= n) sh.srow[l] = v;
         }
         __syncthreads();
+        if (sh.piv_s == 0.0) return 2;  // the same in every wave
         const double pr = (l <= n) ? sh.prow[l] : 0.0;
         const double moved = (irow != icol && l <= n) ? sh.srow[l] : 0.0;
         // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
-        c = {-1.0, 0.0, -1};
+        c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
 #pragma unroll
         for (int k = 0; k < kGJRows; ++k) {
