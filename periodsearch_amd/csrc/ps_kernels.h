@@ -829,9 +829,11 @@ struct PivotCand {
     int q;     // row * n + column, -1 = none
 };
 
+// A lane visits its candidates in increasing index order (rows ascending, its column fixed), so
+// "larger |a|, ties to the larger index" is |a| >= the best so far.
 __device__ inline void pivot_take(PivotCand &c, double s, int q) {
     const double v = fabs(s);
-    if (v > c.v || (v == c.v && q > c.q)) c = {v, q};
+    if (v >= c.v) c = {v, q};
 }
 
 __device__ inline void pivot_merge(PivotCand &c, const PivotCand &o) {
