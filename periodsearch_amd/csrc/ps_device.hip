@@ -2,7 +2,7 @@
 // and launches the gfx950 fit/eval kernels of ps_kernels.h.
 //
 // Data layout in HBM (FP64, structure of arrays):
-//   points  t, ex,ey,ez (Earth), sx,sy,sz (Sun), y, s2i = 1/sig^2, w, sig   [npts = ndata + 3 conv rows]
+//   points  t, ex,ey,ez (Earth), sx,sy,sz (Sun), alpha, y, sig, wq = w/sig^2   [npts = ndata + 3 conv rows]
 //   facets  nx,ny,nz, darea [nfac]; dsph [nfac][dpitch] (zero-padded); pleg [nfac][npairs]; fc, fs
 //   per slot (one per resident workgroup): J [npts][64] raw model rows, ytemp [npts],
 //            dave [nlc][ma+1] relative-lightcurve column sums, alpha [2][mfit^2], beta [2][mfit]
@@ -63,13 +63,13 @@ struct ps_handle {
     bool ls = false;  // the fit needs the Lambert column (cl fitted)
     int ndata = 0, npts = 0, nlc = 0, nfac = 0, ncoef = 0, ma = 0, mfit = 0, nmax = 0, slots = 0;
     double freq_start = 0, freq_end = 0, freq_step = 0, escl = 0, conw_r0 = 0;
-    std::vector<double> s2i_host;  // 1/sig^2 per point (convexity rows set per call)
+    std::vector<double> wq_host;  // (1/sig^2)*w per point (convexity rows, w = 1, set per call)
     double conw_loaded = -1;
     DevProblem P{};
     const DevProblem *dP = nullptr;  // device copy of P: the kernels read its fields on demand
     Slots S{};
     Variant var{};
-    double *d_s2i = nullptr;
+    double *d_wq = nullptr;
     std::vector<void *> owned;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -125,8 +125,8 @@ int dispatch(const Variant &v, bool mix, F &&f) {
 int set_conw(ps_handle *h, double conw_r) {
     if (h->conw_loaded == conw_r) return PS_OK;
     const double sig = 1 / conw_r;  // period_search_BOINC.cpp:721-726
-    for (int j = 0; j < 3; ++j) h->s2i_host[h->ndata + j] = 1 / (sig * sig);
-    HIPCHECK(hipMemcpyAsync(h->d_s2i + h->ndata, h->s2i_host.data() + h->ndata, 3 * sizeof(double),
+    for (int j = 0; j < 3; ++j) h->wq_host[h->ndata + j] = 1 / (sig * sig);
+    HIPCHECK(hipMemcpyAsync(h->d_wq + h->ndata, h->wq_host.data() + h->ndata, 3 * sizeof(double),
                             hipMemcpyHostToDevice, h->stream));
     h->conw_loaded = conw_r;
     return PS_OK;
@@ -192,7 +192,7 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     // points (SoA); the convexity rows have y = 0, weight 1, t/e/e0 unused (:455-456, :721-726)
     std::vector<double> t(npts, 0), ex(npts, 0), ey(npts, 0), ez(npts, 0), sx(npts, 0), sy(npts, 0), sz(npts, 0),
         alpha(npts, 0), y(npts, 0), w(npts, 1), sig(npts, 1);
-    h->s2i_host.assign(npts, 0);
+    h->wq_host.assign(npts, 0);
     for (int p = 0; p < nobs; ++p) {
         t[p] = in->tim[p];
         ex[p] = in->ee[3 * p];
@@ -207,7 +207,9 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
         y[p] = in->brightness[p];
         w[p] = in->weight[p];
         sig[p] = in->sig[p];
-        h->s2i_host[p] = 1 / (in->sig[p] * in->sig[p]);
+        // the weight of the point in alpha, beta and chi2: (1/sig^2)*w, the product mrqcof forms per
+        // evaluation (mrqcof.c:112-113, 121, 134), formed once
+        h->wq_host[p] = (1 / (in->sig[p] * in->sig[p])) * w[p];
     }
     std::vector<int> lc_start(h->nlc + 1), lc_inrel(h->nlc), lc_of(npts);
     lc_start[0] = 0;
@@ -299,10 +301,9 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.sz = dev_upload(sz.data(), npts, o);
     P.alpha = dev_upload(alpha.data(), npts, o);
     P.y = dev_upload(y.data(), npts, o);
-    P.w = dev_upload(w.data(), npts, o);
     P.sig = dev_upload(sig.data(), npts, o);
-    h->d_s2i = dev_upload(h->s2i_host.data(), npts, o);
-    P.s2i = h->d_s2i;
+    h->d_wq = dev_upload(h->wq_host.data(), npts, o);
+    P.wq = h->d_wq;
     P.lc_start = dev_upload(lc_start.data(), lc_start.size(), o);
     P.lc_inrel = dev_upload(lc_inrel.data(), lc_inrel.size(), o);
     P.lc_of = dev_upload(lc_of.data(), lc_of.size(), o);

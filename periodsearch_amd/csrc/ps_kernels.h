@@ -51,7 +51,8 @@ struct DevProblem {
     double iter_diff_max;
     double phi0, alamda_start, alamda_incr;
     double par1, par2, par3, log_cl;
-    const double *t, *ex, *ey, *ez, *sx, *sy, *sz, *y, *s2i, *w, *sig;
+    const double *t, *ex, *ey, *ez, *sx, *sy, *sz, *y, *sig;
+    const double *wq;     // weight/sig^2 of each point
     const double *alpha;  // phase angle of each point, acos(ee . ee0)
     const int *lc_start, *lc_inrel, *lc_of;
     // phase B tiles (16 observed points each) split into runs of one lightcurve ("segments"):
@@ -636,10 +637,18 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     phase_terms(P, sh.a, ytemp, J + kRes, kMap);
     if (tid == 0) sh.next_tp = kWaves;  // tile pairs 0..7 go to waves 0..7
     if (LDSB) {
-        if constexpr (MIX)
-            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsphf[i] = gload(P.dsphf, i);
-        else
-            for (int i = tid; i < P.nf * P.dpitch; i += kWG) sh.u.dsph[i] = gload(P.dsph, i);
+        // a loop of uniform trip count: its exit runs with every lane active (register spills placed
+        // at the exit of a divergent loop run before the lanes are restored)
+        const int n = P.nf * P.dpitch;
+        for (int i0 = 0; i0 < n; i0 += kWG) {
+            const int i = i0 + tid;
+            if (i < n) {
+                if constexpr (MIX)
+                    sh.u.dsphf[i] = gload(P.dsphf, i);
+                else
+                    sh.u.dsph[i] = gload(P.dsph, i);
+            }
+        }
     }
     if constexpr (MIX)
         for (int i = tid; i < P.nf; i += kWG) {
@@ -722,7 +731,7 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         const size_t pp = g.in ? p : 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) g.raw[u] = gload(J, pp * kMap + coll[u]);
-        g.wq = gload(P.s2i, pp) * gload(P.w, pp);
+        g.wq = gload(P.wq, pp);
         g.yt = gload(ytemp, pp);
         g.sig = gload(P.sig, pp);
         g.y = gload(P.y, pp);
@@ -739,13 +748,20 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         colt[u] = kind[u] == 3 ? col : 0;
     }
     auto stage2 = [&](const RowG &g, RowV &r) {
-        const bool rel = rell[g.L];
-        const double cf = cfl[g.L];
-        const double coef = rel ? g.sig * cf : 1.0;
+        // The table reads are issued together and pinned by an empty asm that takes their values: left
+        // alone, the compiler moves each read into a branch over relative curves and waits for them one
+        // at a time.
+        int reli = rell[g.L];
+        double cf = cfl[g.L], mv[4];
         const double *m = tab + g.L * tpitch;
 #pragma unroll
+        for (int u = 0; u < 4; ++u) mv[u] = m[colt[u]];
+        asm volatile("" : "+v"(reli), "+v"(cf), "+v"(mv[0]), "+v"(mv[1]), "+v"(mv[2]), "+v"(mv[3]));
+        const bool rel = reli != 0;
+        const double coef = rel ? g.sig * cf : 1.0;
+#pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double xr = coef * (g.raw[u] - g.yt * m[colt[u]]);
+            const double xr = coef * (g.raw[u] - g.yt * mv[u]);
             const double xo = rel ? ((kind[u] == 2) ? 0.0 : xr) : g.raw[u];
             const double x = (kind[u] == 1) ? g.y - coef * g.yt : xo;
             r.v[u] = (g.in && kind[u] != 0) ? x : 0.0;
@@ -760,26 +776,31 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
             for (int bj = 0; bj <= bi; ++bj, ++idx)  // blocks beyond nb add zeros
                 acc[idx] = Pd::mma(r.v[bi], r.wq * r.v[bj], acc[idx]);
     };
-    // Three load buffers, unrolled by three so that none is copied (a register copy of fresh load data
-    // would wait for it): the loads of step t + 3 are issued while step t + 1 is normalised and step t
+    // Four load buffers, unrolled by four so that none is copied (a register copy of fresh load data
+    // would wait for it): the loads of step t + 4 are issued while step t + 1 is normalised and step t
     // runs on the MFMA.  Steps past the end load row 0 and contribute zeros.
     constexpr int K = kWaves;
-    RowG ga, gb, gc;
+    RowG ga, gb, gc, gd;
     RowV cur, nxt;
     stage1(wave, ga);
     stage1(wave + K, gb);
     stage1(wave + 2 * K, gc);
+    stage1(wave + 3 * K, gd);
     stage2(ga, cur);
-    for (int s = wave; s < nsteps; s += 3 * K) {
-        stage1(s + 3 * K, ga);
+    for (int s = wave; s < nsteps; s += 4 * K) {
+        stage1(s + 4 * K, ga);
         stage2(gb, nxt);
         accumulate(cur);
         cur = nxt;
-        stage1(s + 4 * K, gb);
+        stage1(s + 5 * K, gb);
         stage2(gc, nxt);
         accumulate(cur);
         cur = nxt;
-        stage1(s + 5 * K, gc);
+        stage1(s + 6 * K, gc);
+        stage2(gd, nxt);
+        accumulate(cur);
+        cur = nxt;
+        stage1(s + 7 * K, gd);
         stage2(ga, nxt);
         accumulate(cur);
         cur = nxt;
