@@ -378,6 +378,9 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int pl = lane & 15, kg = lane >> 4;
     const int nc = P.nc, pitch = P.dpitch;
+    // With leftover columns (R > 0) the layout is used for nc = 16 NT + R exactly, so every column of
+    // it is a coefficient; with R == 0 (nc <= 16 NT) the columns from nc on are padding.
+    auto is_coef = [&](int col) { return R > 0 || col < nc; };
     const double *a = sh.a;
     const T *dsph, *snx, *sny, *snz, *sar, *sac, *sas;
     if constexpr (MIX) {
@@ -492,7 +495,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) {
                     val[tt][r] = (pi < P.nobs) ? sc * double(ts[h].acc[tt][r]) : 0.0;
-                    if (pi < P.nobs && 16 * tt + pl < nc) jr[16 * tt + pl] = val[tt][r];
+                    if (pi < P.nobs && is_coef(16 * tt + pl)) jr[16 * tt + pl] = val[tt][r];
                 }
             }
             // per-lightcurve column sums of this tile's rows, one partial row per segment
@@ -508,7 +511,7 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                         x += (i >= q0 && i < q1) ? val[tt][r] : 0.0;
                     }
                     x = sum_groups(x);
-                    if (kg == 0 && 16 * tt + pl < nc) part[size_t(sg) * kMap + 16 * tt + pl] = x;
+                    if (kg == 0 && is_coef(16 * tt + pl)) part[size_t(sg) * kMap + 16 * tt + pl] = x;
                 }
             }
         }
@@ -554,18 +557,23 @@ __device__ __forceinline__ void model_rows(const DevProblem &P, Shared &sh, doub
                 double *jr = J + size_t(p) * kMap;
 #pragma unroll
                 for (int k = 0; k < kOther - 1; ++k)
-                    if (k >= R || 16 * NT + k < nc) jr[ocol(k)] = ov[k];
+                    if (k >= R || is_coef(16 * NT + k)) jr[ocol(k)] = ov[k];
                 ytemp[p] = ov[kOther - 1];
             }
+            // per-segment column sums: the four 16-lane rows hold the same values, so row kg sums
+            // column 4j + kg, four columns per reduction
             for (int sg = uload(P.seg_first, tile); sg < uload(P.seg_first, tile + 1); ++sg) {
                 if (!uload(P.lc_inrel, uload(P.seg_lc, sg))) continue;
                 const int rg = uload(P.seg_range, sg), q0 = rg & 255, q1 = rg >> 8;
                 const bool in = valid[h] && pl >= q0 && pl < q1;
 #pragma unroll
-                for (int k = 0; k < kOther; ++k) {
-                    if (k < R && 16 * NT + k >= nc) continue;
-                    const double x = sum_row16(in ? ov[k] : 0.0);
-                    if (lane == 0) part[size_t(sg) * kMap + ocol(k)] = x;
+                for (int j = 0; 4 * j < kOther; ++j) {
+                    double v = 0;
+#pragma unroll
+                    for (int q = 0; q < 4 && 4 * j + q < kOther; ++q) v = (kg == q) ? ov[4 * j + q] : v;
+                    const double x = sum_row16(in ? v : 0.0);
+                    const int k = 4 * j + kg;
+                    if (pl == 0 && k < kOther && (k >= R || is_coef(16 * NT + k))) part[size_t(sg) * kMap + ocol(k)] = x;
                 }
             }
         }
