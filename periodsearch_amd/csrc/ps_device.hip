@@ -156,9 +156,10 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
                        "GPU kernels support up to %d fitted of %d parameters (got %d of %d; Lmax <= 6)", kMaxFit,
                        psk::kRes, mfit, ma);
     if (geo->nfac > kMaxFac) return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support nrows <= 8 (nfac <= %d)", kMaxFac);
-    // the per-lightcurve normalisation table (nlc x (ma + 1) sums + nlc factors) lives in LDS
-    if ((in->lcurves + 1) * (ma + 3) > psk::kDsphLds)
-        return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support up to %d lightcurves at ma = %d", psk::kDsphLds / (ma + 3) - 1,
+    // the per-lightcurve normalisation table (nlc x (ma + 1) sums + two factors and a flag per curve)
+    // lives in LDS
+    if ((in->lcurves + 1) * (ma + 4) > psk::kDsphLds)
+        return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support up to %d lightcurves at ma = %d", psk::kDsphLds / (ma + 4) - 1,
                        ma);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ps_fail(PS_ERR_DEVICE, "no HIP device");

@@ -676,12 +676,12 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
     }
     // Phase C: per relative lightcurve L the column sums dave_l = sum dyda_l and ave = sum ymod over
     // its points (mrqcof.c:53-87), added up from the per-segment partial rows of phase B, into an LDS
-    // table in the union (Dsph is no longer needed): tab[L][l] = dave_l / ave for l < ma,
-    // tab[L][ma] = ave, cfl[L] = Lpoints / ave and rell[L] = 1 for relative curves (ave, and so cfl, can
-    // have either sign: a trial step may turn the phase function negative).
+    // table in the union (Dsph is no longer needed): tab[L][l] = dave_l for l < ma, tab[L][ma] = ave,
+    // cfl[L] = Lpoints / ave, rav[L] = 1 / ave and rell[L] = 1 for relative curves (ave, and so cfl,
+    // can have either sign: a trial step may turn the phase function negative).
     const int tpitch = ma + 1;
-    double *tab = sh.u.dsph, *cfl = sh.u.dsph + P.nlc * tpitch;
-    int *rell = reinterpret_cast<int *>(cfl + P.nlc);
+    double *tab = sh.u.dsph, *cfl = sh.u.dsph + P.nlc * tpitch, *rav = cfl + P.nlc;
+    int *rell = reinterpret_cast<int *>(rav + P.nlc);
     for (int it = tid; it < P.nlc * tpitch; it += kWG) {
         const int L = it / tpitch, l = it - L * tpitch;
         double s = 0;
@@ -690,23 +690,22 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         tab[it] = s;
     }
     __syncthreads();
-    for (int it = tid; it < P.nlc * tpitch; it += kWG) {
-        const int L = it / tpitch, l = it - L * tpitch;
+    for (int L = tid; L < P.nlc; L += kWG) {
         const bool rel = P.lc_inrel[L];
-        if (l < ma) {
-            if (rel) tab[it] = tab[it] / tab[L * tpitch + ma];
-        } else {
-            cfl[L] = rel ? double(P.lc_start[L + 1] - P.lc_start[L]) / tab[it] : 0.0;
-            rell[L] = rel;
-        }
+        const double ave = tab[L * tpitch + ma];
+        cfl[L] = rel ? double(P.lc_start[L + 1] - P.lc_start[L]) / ave : 0.0;
+        rav[L] = rel ? 1 / ave : 0.0;
+        rell[L] = rel;
     }
     __syncthreads();
     // Phase D: alpha/beta/chi2 (mrqcof.c:89-145).  Each wave takes 4-point K-steps; lane (pl, kg) holds
     // row 4s + kg, column 16u + pl of the normalised fitted columns plus the residual as column mfit,
     // which is at once the MFMA A operand (column x point) and, times w/sig^2, the B operand.  A row
-    // of relative lightcurve L is normalised as it is loaded: coef*(dyda_l - ymod*dave_l/ave) with
+    // of relative lightcurve L is normalised as it is loaded: coef*(dyda_l - (ymod*dave_l)/ave) with
     // coef = sig*Lpoints/ave, dyda for c00 = 0, residual y - coef*ymod; other rows keep dyda and have
-    // residual y - ymod.
+    // residual y - ymod.  ymod*dave_l is formed first, as the reference does (mrqcof.c:99): where it
+    // overflows (a runaway Lambert weight), the reference's normal equations hold inf and its
+    // Gauss-Jordan fails, and so do these.
     const int nb = (mfit + 16) >> 4;  // 16-column blocks covering mfit + 1 columns
     int colu[4];
 #pragma unroll
@@ -760,16 +759,16 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
         // alone, the compiler moves each read into a branch over relative curves and waits for them one
         // at a time.
         int reli = rell[g.L];
-        double cf = cfl[g.L], mv[4];
+        double cf = cfl[g.L], ra = rav[g.L], mv[4];
         const double *m = tab + g.L * tpitch;
 #pragma unroll
         for (int u = 0; u < 4; ++u) mv[u] = m[colt[u]];
-        asm volatile("" : "+v"(reli), "+v"(cf), "+v"(mv[0]), "+v"(mv[1]), "+v"(mv[2]), "+v"(mv[3]));
+        asm volatile("" : "+v"(reli), "+v"(cf), "+v"(ra), "+v"(mv[0]), "+v"(mv[1]), "+v"(mv[2]), "+v"(mv[3]));
         const bool rel = reli != 0;
         const double coef = rel ? g.sig * cf : 1.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double xr = coef * (g.raw[u] - g.yt * mv[u]);
+            const double xr = coef * (g.raw[u] - (g.yt * mv[u]) * ra);
             const double xo = rel ? ((kind[u] == 2) ? 0.0 : xr) : g.raw[u];
             const double x = (kind[u] == 1) ? g.y - coef * g.yt : xo;
             r.v[u] = (g.in && kind[u] != 0) ? x : 0.0;

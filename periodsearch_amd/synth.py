@@ -46,6 +46,7 @@ class SynthSpec:
     conw: float = 0.1
     phase: tuple = (0.5, 0.1, -0.5)
     cl: float = 0.1
+    cl_free: bool = False  # fit the Lambert weight too (the input's cl flag)
     extra: dict = field(default_factory=dict)
 
 
@@ -158,7 +159,7 @@ def generate(spec: SynthSpec):
     out.append(f"{spec.phase[0]} 1\t\tinitial guesses for phase funct. params.")
     out.append(f"{spec.phase[1]} 1")
     out.append(f"{spec.phase[2]} 1")
-    out.append(f"{spec.cl} 0\t\tinitial Lambert coeff. (L-S=1)")
+    out.append(f"{spec.cl} {int(spec.cl_free)}\t\tinitial Lambert coeff. (L-S=1)")
     out.append(f"{spec.stop_condition:g}\t\tstop condition")
     out.append(f"{spec.n_iter_min}\t\tminimum number of iterations")
     out.append("5\t\tAlamda multiplicative factor")
