@@ -106,8 +106,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the scan shards by "
-              "independent period ranges, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); a scan over several GPUs runs one "
+              "process per GPU on independent ranges of trial periods", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
