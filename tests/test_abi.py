@@ -103,3 +103,21 @@ def test_device_calls_need_prepare():
     ps = PeriodSearch.from_text(text)
     with pytest.raises(RuntimeError):
         ps.fit_batch([0.14], [1], 0.1)
+
+
+def test_prepare_rejects_shapes_beyond_the_limits():
+    """Shape limits are checked before a device is touched.  The reference's own limit of 200
+    lightcurves (MAX_LC, period_search_BOINC.cpp:324-325; PS_ERR_LIMIT) binds before the kernels' LDS
+    normalisation table, which holds (lightcurves + 1) * (ma + 4) <= 288 * 49 doubles (230 lightcurves
+    at Lmax 6, ma = 57); 200 lightcurves pass both (and, without a GPU here, fail on the device).  Lmax 7 (ma = 72 > 63) is
+    beyond the kernels (PS_ERR_UNSUPPORTED)."""
+    codes = {k: v for v, k in L.ERRORS.items()}
+
+    def code(**kw):
+        text, _ = synth.generate(synth.config0(dense_points=3, **kw))
+        with pytest.raises(PsError) as e:
+            PeriodSearch.from_text(text).prepare(device=0)
+        return e.value.code
+    assert code(fit_lmax=6, n_dense=201) == codes["PS_ERR_LIMIT"]
+    assert code(fit_lmax=6, n_dense=200) not in (codes["PS_ERR_LIMIT"], codes["PS_ERR_UNSUPPORTED"])
+    assert code(fit_lmax=7, n_dense=4) == codes["PS_ERR_UNSUPPORTED"]
