@@ -161,6 +161,12 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     if ((in->lcurves + 1) * (ma + 4) > psk::kDsphLds)
         return ps_fail(PS_ERR_UNSUPPORTED, "GPU kernels support up to %d lightcurves at ma = %d", psk::kDsphLds / (ma + 4) - 1,
                        ma);
+    // the kernels take every column of a layout with leftover VALU columns (R > 0) as a coefficient
+    {
+        const Variant v = pick_variant(ncoef, geo->nfac);
+        if ((v.r > 0 && 16 * v.nt + v.r != ncoef) || 16 * v.nt + v.r < ncoef)
+            return ps_fail(PS_ERR_UNSUPPORTED, "no kernel layout for %d shape coefficients", ncoef);
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ps_fail(PS_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return ps_fail(PS_ERR_ARG, "device %d out of range", device);
