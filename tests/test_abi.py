@@ -107,7 +107,7 @@ def test_device_calls_need_prepare():
 
 def test_prepare_rejects_shapes_beyond_the_limits():
     """Shape limits are checked before a device is touched.  The reference's own limit of 200
-    lightcurves (MAX_LC, period_search_BOINC.cpp:324-325; PS_ERR_LIMIT) binds before the kernels' LDS
+    lightcurves (MAX_LC, period_search_BOINC.cpp:324-327; PS_ERR_LIMIT) binds before the kernels' LDS
     normalisation table, which holds (lightcurves + 1) * (ma + 4) <= 288 * 49 doubles (230 lightcurves
     at Lmax 6, ma = 57); 200 lightcurves pass both (and, without a GPU here, fail on the device).  Lmax 7 (ma = 72 > 63) is
     beyond the kernels (PS_ERR_UNSUPPORTED)."""
