@@ -122,6 +122,35 @@ def test_fit_free_lambert_and_layouts_match_oracle(lmax, cl_free):
         ps.close()
 
 
+# ---- the convergence stop (stop condition < 1, period_search_BOINC.cpp:856-860 and 868-892: up to
+# MAX_N_ITER = 100 iterations while the rms still improves by more than the condition, at least the
+# input's minimum): both exits, converged
+# after 5-8 iterations and the 100-iteration cap, with the same iteration count as the oracle
+@pytest.mark.parametrize("stop", [0.01, 0.001])
+def test_fit_convergence_stop_matches_oracle(stop):
+    text, _ = synth.generate(synth.config0(n_dense=4, dense_points=30, stop_condition=stop, n_iter_min=5))
+    orc = ob.OracleCtx.from_text(text)
+    ps = PeriodSearch.from_text(text).prepare(device=0)
+    try:
+        conw_r = ps.info()["conw_r0"]
+        orc.set_conw(conw_r)
+        n = np.array([1, 1, 2, 3, 5])
+        poles = np.array([1, 6, 3, 9, 2])
+        got = ps.fit_batch(ps.freq(n), poles, conw_r)
+        iters = []
+        for i in range(len(n)):
+            ref = orc.fit(orc.freq(int(n[i])), int(poles[i]))
+            iters.append(ref["niter"])
+            assert got["niter"][i] == ref["niter"]
+            assert _rel(got["dev"][i], ref["dev_new"]) < 1e-9
+            assert _rel(got["chisq"][i], ref["chisq"]) < 1e-9
+            assert _rel(got["prd"][i], ref["prd"]) < 1e-9
+        if stop == 0.001:
+            assert min(iters) < 100 and max(iters) == 100  # both exits taken
+    finally:
+        ps.close()
+
+
 def test_scan_and_conw_match_oracle(small):
     ps, orc = small
     cw_gpu, nc_gpu = ps.conw_search()
