@@ -1,14 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the period scan hot path: trial-period Levenberg-Marquardt fits per second at
 Lmax = Mmax = 6 on 288 facets (the production shape: 40 dense lightcurves x 50 points + 400 sparse
-calibrated points, ndata = 2400, 55 fitted parameters, stop condition 50 = fixed 50 LM iterations).
+calibrated points, ndata = 2400, 55 fitted parameters, stop condition 50 = fixed 50 LM iterations),
+FP64 and mixed precision on the same fits, with the best-period agreement of the two.
 
 A step is one ps_fit_batch call over a window of trial periods x 10 initial poles, inputs resident
 in HBM (uploaded once by ps_prepare); ps_fit_batch returns after the library's stream has
-synchronised, so the wall clock around the K steps covers all device work.  The roofline figure uses
-the fit kernel's own duration, measured with HIP events recorded on the stream it is launched on.
+synchronised, so the wall clock around the K steps covers all device work.  Step k scans the k-th
+window of the grid, the first one centred on the true period.  The default run times the FP64
+kernels (the line's value) and then the mixed-precision kernels on the same windows; it reports the
+mixed rate, the speedup, whether both precisions pick the same best period in every window and the
+per-period chi2 difference.  The roofline figures use the fit kernel's own duration, measured with
+HIP events recorded on the stream it is launched on.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--periods P] [--workload config1|config3]
+    python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed] [--periods P]
+                    [--workload config1|config3]
 
 The default workload (config1) is the headline; config3 (a sparse survey fitted on 512 facets, Dsph
 read through L2) is a second shape, reported under its own metric name.
@@ -26,16 +32,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "trial-period LM fits/sec at Lmax=6, 288 facets: mixed vs FP64, same best period"
 METRIC_CONFIG3 = "trial-period LM fits/sec at Lmax=6, 512 facets, sparse survey (5000 points)"
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
+PEAK_FP64 = 78.6   # MI355X dense FP64 matrix peak, TFLOP/s
+PEAK_FP32 = 157.3  # MI355X dense FP32 matrix peak, TFLOP/s
 ORACLE_CLI = os.path.join(ROOT, "oracle", "ps_oracle_cli")
 REF_CLI = os.path.join(ROOT, "oracle", "_ref", "ps_ref_cli")
+REF_FMA_CLI = os.path.join(ROOT, "oracle", "_ref", "ps_ref_fma_cli")
+# the host CPU share of one GPU on the benchmark machines: worker pools stay within it
+MAX_CPU_WORKERS = 16
 
 
-def algorithmic_flops_per_eval(info):
-    """Shape-Jacobian contraction + J^T W J lower triangle with diagonal + J^T W r, per model
-    evaluation: N * (2 Nf Nc + mfit (mfit + 1) + 2 mfit), N = ndata + 3 convexity rows."""
+def flops_per_eval(info):
+    """Algorithmic flops of one model evaluation, N = ndata + 3 convexity rows: the shape-Jacobian
+    contraction N * 2 Nf Nc, and the normal equations N * (mfit (mfit + 1) + 2 mfit) (the J^T W J lower
+    triangle with diagonal, and J^T W r)."""
     n, nf, nc, mf = info["npts"], info["nfac"], info["ncoef"], info["mfit"]
-    return n * (2 * nf * nc + mf * (mf + 1) + 2 * mf)
+    return n * 2 * nf * nc, n * (mf * (mf + 1) + 2 * mf)
 
 
 def traffic_per_launch(workload, precision):
@@ -43,23 +54,21 @@ def traffic_per_launch(workload, precision):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        t = json.load(f)
-    return t.get("hbm_bytes_per_launch")
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(input_path, conw_r, seconds_target=20.0):
-    """The reference's own plain-C core (oracle/_ref/ps_ref_cli, compiled from the reference sources by
-    oracle/Makefile; kind "reference") or, where that was not built, the oracle's FP64 restatement
-    (oracle/ps_oracle_cli; kind "port"), single-threaded as the reference is, one process per host core
-    (at most 16) on disjoint trial periods of the same workload -- the reference's own data-parallel
-    split.  Each process runs whole periods (10 poles) for about seconds_target."""
-    if os.path.exists(REF_CLI):
-        cli, kind = REF_CLI, "reference"
-    elif os.path.exists(ORACLE_CLI):
-        cli, kind = ORACLE_CLI, "port"
-    else:
+def cpu_cores():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(MAX_CPU_WORKERS, n))
+
+
+def cpu_baseline(cli, label, kind, input_path, conw_r, seconds_target):
+    """A single-threaded CPU build of the fit, one process per host core (at most the GPU's CPU share),
+    on disjoint trial periods of the same workload -- the reference's own data-parallel split.  Each
+    process runs whole periods (10 poles) for about seconds_target."""
+    if not os.path.exists(cli):
         return None
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    cores = cpu_cores()
     probe = subprocess.run([cli, "fits", input_path, "1", "1", repr(conw_r), "1"], capture_output=True, text=True,
                            check=True)
     t_fit = float(probe.stdout.split()[3])
@@ -75,7 +84,7 @@ def cpu_baseline(input_path, conw_r, seconds_target=20.0):
         nfits += int(out.split()[1])
     wall = time.perf_counter() - t0
     return {"value": round(nfits / wall, 3), "unit": "fits/s", "cores": cores, "kind": kind,
-            "sample": f"{cores} processes x {nper} trial period(s) x 10 poles of the same workload "
+            "sample": f"{label}: {cores} processes x {nper} trial period(s) x 10 poles of the same workload "
                       f"({nfits} fits, {wall:.1f} s wall), {os.path.relpath(cli, ROOT)}"}
 
 
@@ -93,19 +102,68 @@ def copy_calibration(mib):
           flush=True)
 
 
+def run_precision(ps, precision, windows, warmup, conw_r):
+    """W untimed warmup steps (window 0), then one timed step per window."""
+    import numpy as np
+    ps.set_precision(precision)
+    pole = None
+
+    def step(n):
+        nonlocal pole
+        if pole is None or len(pole) != 10 * len(n):
+            pole = 1 + np.arange(10 * len(n)) % 10
+        return ps.fit_batch(ps.freq(np.repeat(n, 10)), pole, conw_r)
+
+    for _ in range(warmup):
+        step(windows[0])
+    ps.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    res = [step(n) for n in windows]
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = ps.kernel_time()
+    fits = 10 * sum(len(n) for n in windows)
+    return {"fits": fits, "wall": wall, "value": fits / wall, "avg_launch_s": kernel_ms * 1e-3 / launches,
+            "results": res}
+
+
+def best_periods(res, n, ndata):
+    """Per trial period the best of the 10 poles (lowest rms, first pole wins ties;
+    period_search_BOINC.cpp:913-920): its rms and the line's chi2 = rms^2 * ndata."""
+    import numpy as np
+    dev = res["dev"].reshape(len(n), 10)
+    best = dev[np.arange(len(n)), np.argmin(dev, axis=1)]
+    return best, best * best * ndata
+
+
+def agreement(r64, rmx, windows, ndata):
+    import numpy as np
+    same, errs, fit_errs = 0, [], []
+    for a, b, n in zip(r64, rmx, windows):
+        d64, c64 = best_periods(a, n, ndata)
+        dmx, cmx = best_periods(b, n, ndata)
+        same += int(np.argmin(d64) == np.argmin(dmx))
+        errs.append(np.abs(cmx - c64) / c64)
+        fit_errs.append(np.abs(b["chisq"] - a["chisq"]) / a["chisq"])
+    errs, fit_errs = np.concatenate(errs), np.concatenate(fit_errs)
+    return {"windows": len(windows), "same_best_period_windows": same,
+            "period_chi2_rel_err_median": float(f"{np.median(errs):.3g}"),
+            "period_chi2_rel_err_max": float(f"{np.max(errs):.3g}"),
+            "fit_chi2_rel_err_median": float(f"{np.median(fit_errs):.3g}")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--compare", action="store_true",
-                    help="also run the other precision on the same fits: speedup and best-period agreement")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="both (default): FP64 as the value, mixed on the same fits beside it")
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
     ap.add_argument("--workload", choices=["config1", "config3"], default="config1",
                     help="config1: dense photometry, Lmax 6 on 288 facets, 2400 points (the headline); "
                          "config3: sparse survey, Lmax 6 on 512 facets, 5000 points")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline process")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale factors)")
     args = ap.parse_args()
@@ -121,63 +179,78 @@ def main():
 
     from periodsearch_amd import PS_FP64, PS_MIXED, PeriodSearch, synth
 
-    text, _ = synth.generate(getattr(synth, args.workload)())
+    text, truth = synth.generate(getattr(synth, args.workload)())
     tmpdir = tempfile.mkdtemp(prefix="psbench_")
     input_path = os.path.join(tmpdir, "period_search_in")
     with open(input_path, "w") as f:
         f.write(text)
-    ps = PeriodSearch.from_file(input_path).prepare(device=0,
-                                                    precision=PS_MIXED if args.precision == "mixed" else PS_FP64)
+    ps = PeriodSearch.from_file(input_path).prepare(device=0)
     info = ps.info()
     conw_r = info["conw_r0"]
-    per_step = args.periods * 10
+    # step k scans window k of the grid; window 0 is centred on the true period
+    n_true = int(round((info["freq_start"] - 1 / truth["period_h"]) / info["freq_step"])) + 1
+    n0 = max(1, n_true - args.periods // 2)
+    windows = [1 + (n0 - 1 + k * args.periods + np.arange(args.periods)) % info["nmax"] for k in range(args.steps)]
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: run_precision(ps, PS_FP64 if p == "fp64" else PS_MIXED, windows, args.warmup, conw_r)
+            for p in precisions}
+    ps.close()
 
-    def step(k):
-        n = 1 + (k * args.periods + np.arange(per_step) // 10) % info["nmax"]
-        pole = 1 + np.arange(per_step) % 10
-        return ps.fit_batch(ps.freq(n), pole, conw_r)
-
-    for k in range(args.warmup):
-        step(k)
-    ps.kernel_time(reset=True)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        res = step(args.warmup + k)
-    wall = time.perf_counter() - t0
-    kernel_ms, launches = ps.kernel_time()
-    fits = per_step * args.steps
-    value = fits / wall
-
-    niter = float(np.mean(res["niter"]))
-    f_fit = (niter + 1) * algorithmic_flops_per_eval(info)
-    avg_launch_s = kernel_ms * 1e-3 / launches
-    achieved = per_step * f_fit / avg_launch_s / 1e12
-    peak = PEAK_TFLOPS[args.precision]
+    head = precisions[0]
+    r = runs[head]
+    niter = float(np.mean(np.concatenate([x["niter"] for x in r["results"]])))
+    f_shape, f_ne = flops_per_eval(info)
     workload = f"{args.workload}_lmax6_{info['nfac']}fac_ndata{info['ndata']}"
+    per_step = 10 * args.periods
+
+    def roofline(p):
+        x = runs[p]
+        f_fit = (niter + 1) * (f_shape + f_ne)
+        achieved = per_step * f_fit / x["avg_launch_s"] / 1e12
+        if p == "fp64":
+            peak = PEAK_FP64
+        else:  # shape contraction on FP32 MFMA, normal equations on FP64 MFMA: the time-weighted peak
+            peak = (f_shape + f_ne) / (f_shape / PEAK_FP32 + f_ne / PEAK_FP64)
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": traffic_per_launch(workload, p),
+                "flops_per_fit": f_fit, "avg_launch_ms": round(x["avg_launch_s"] * 1e3, 3)}
+
     line = {
         "metric": METRIC if args.workload == "config1" else METRIC_CONFIG3,
-        "value": round(value, 2),
+        "value": round(r["value"], 2),
         "unit": "fits/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": f"synthetic (seeded generator, periodsearch_amd/synth.py {args.workload})",
         "config": {"workload": workload, "ndata": info["ndata"], "lmax": 6, "nfac": info["nfac"],
                    "mfit": info["mfit"], "lm_iterations": niter, "periods_per_step": args.periods,
-                   "fits_per_step": per_step, "precision": args.precision,
-                   "trial_periods_per_s": round(value / 10, 2)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4),
-                     "traffic": traffic_per_launch(workload, args.precision),
-                     "flops_per_fit": f_fit, "avg_launch_ms": round(avg_launch_s * 1e3, 3)},
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(input_path, conw_r),
+                   "fits_per_step": per_step, "precision": head, "first_period_index": int(windows[0][0]),
+                   "trial_periods_per_s": round(r["value"] / 10, 2)},
+        "roofline": roofline(head),
     }
-    ps.close()
+    if len(precisions) == 2:
+        m = runs["mixed"]
+        line["mixed"] = {"value": round(m["value"], 2), "unit": "fits/s",
+                         "ms_per_step": round(m["wall"] / args.steps * 1e3, 3),
+                         "speedup_vs_fp64": round(m["value"] / r["value"], 3), "roofline": roofline("mixed"),
+                         "agreement": agreement(r["results"], m["results"], windows, info["ndata"])}
+    if not args.no_cpu_baseline:
+        fma = cpu_baseline(REF_FMA_CLI, "reference AVX-FMA build", "reference", input_path, conw_r,
+                           args.cpu_seconds)
+        plain = cpu_baseline(REF_CLI, "reference plain-C build", "reference", input_path, conw_r, args.cpu_seconds)
+        if plain is None:
+            plain = cpu_baseline(ORACLE_CLI, "FP64 restatement of the plain-C core", "port", input_path, conw_r,
+                                 args.cpu_seconds)
+        line["cpu_baseline"] = fma or plain
+        line["cpu_baseline_plain"] = plain if fma else None
+    else:
+        line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
 

@@ -1,7 +1,9 @@
 // period_search_amd -- command-line replacement for the reference period_search executable
 // (period_search/period_search_BOINC.cpp:147-954): reads period_search_in, scans the period interval
 // on the GPU and writes period_search_out (with period_search_state checkpoints) in the reference
-// format.  Optional flags: --device N, --precision fp64|mixed, and explicit file names.
+// format.  Optional flags: --device N, --precision fp64|mixed, and explicit file names.  Exit status:
+// 0 on success, 1 when a file cannot be read or written, 2 on any other failure (usage, a malformed
+// input, a reference limit, an unsupported shape, no device).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,7 +30,9 @@ int main(int argc, char **argv) {
     int rc = ps_run_files(in, out, state, device, precision);
     if (rc != PS_OK) {
         std::fprintf(stderr, "period_search_amd: %s (code %d)\n", ps_last_error(), rc);
-        return 2;
+        // the reference exits with 1 when its output cannot be opened (period_search_BOINC.cpp:252-261) and
+        // with 2 on an input limit (:327, :353, :361) or a failed device setup (CUDAPrepare, :608-613)
+        return rc == PS_ERR_IO ? 1 : 2;
     }
     return 0;
 }

@@ -137,7 +137,22 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     if (!in || !geo || !out) return ps_fail(PS_ERR_ARG, "ps_prepare: null argument");
     *out = nullptr;
     const int ncoef = geo->ncoef, ma = ncoef + 8;
-    int ia[kMaxMa + 8];
+    if (ncoef < 1) return ps_fail(PS_ERR_ARG, "ps_prepare: geometry has %d shape coefficients", ncoef);
+    // a caller may fill ps_input itself: the lightcurve table must describe exactly the ndata points
+    // the arrays hold (the kernels index points through it)
+    if (in->ndata < 1 || in->lcurves < 1 || !in->lpoints || !in->inrel || !in->tim || !in->brightness || !in->sig ||
+        !in->weight || !in->ee || !in->ee0)
+        return ps_fail(PS_ERR_ARG, "ps_prepare: input has no points or a null array");
+    {
+        long long total = 0;
+        for (int L = 0; L < in->lcurves; ++L) {
+            if (in->lpoints[L] < 1) return ps_fail(PS_ERR_ARG, "ps_prepare: lightcurve %d has %d points", L, in->lpoints[L]);
+            total += in->lpoints[L];
+        }
+        if (total != in->ndata)
+            return ps_fail(PS_ERR_ARG, "ps_prepare: lightcurves hold %lld points, ndata is %d", total, in->ndata);
+    }
+    std::vector<int> ia(ma);
     for (int i = 0; i < ncoef; ++i) ia[i] = 1;
     if (in->onlyrel == 1) ia[0] = 0;  // period_search_BOINC.cpp:788-802
     ia[ncoef] = 1;

@@ -121,3 +121,27 @@ def test_prepare_rejects_shapes_beyond_the_limits():
     assert code(fit_lmax=6, n_dense=201) == codes["PS_ERR_LIMIT"]
     assert code(fit_lmax=6, n_dense=200) not in (codes["PS_ERR_LIMIT"], codes["PS_ERR_UNSUPPORTED"])
     assert code(fit_lmax=7, n_dense=4) == codes["PS_ERR_UNSUPPORTED"]
+    # Lmax 8 and 10 (ma = 89, 129: the reference's MAX_LM = 10 is the parser's limit) are rejected the same
+    # way, before any per-parameter table is filled
+    assert code(fit_lmax=8, n_dense=4) == codes["PS_ERR_UNSUPPORTED"]
+    assert code(fit_lmax=10, n_dense=4) == codes["PS_ERR_UNSUPPORTED"]
+
+
+def test_prepare_rejects_an_inconsistent_lightcurve_table():
+    """A caller that fills ps_input itself must describe exactly ndata points with the lightcurve
+    lengths (the kernels index points through that table): a mismatch is PS_ERR_ARG, found before a
+    device is touched."""
+    codes = {k: v for v, k in L.ERRORS.items()}
+    text, _ = synth.generate(synth.config0(n_dense=3, dense_points=10))
+    ps = PeriodSearch.from_text(text)
+    ps.input.lpoints[0] += 1
+    with pytest.raises(PsError) as e:
+        ps.prepare(device=0)
+    assert e.value.code == codes["PS_ERR_ARG"]
+    ps.input.lpoints[0] -= 1  # consistent again, but one lightcurve empty
+    ps.input.lpoints[0] += ps.input.lpoints[1]
+    ps.input.lpoints[1] = 0
+    with pytest.raises(PsError) as e:
+        ps.prepare(device=0)
+    assert e.value.code == codes["PS_ERR_ARG"]
+    ps.close()
