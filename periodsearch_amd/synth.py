@@ -201,3 +201,36 @@ def config3(**kw):
                 per_end=100.0, fit_lmax=6, fit_nrows=8)
     base.update(kw)
     return SynthSpec(**base)
+
+
+# The batched uneven workload: many asteroids of different sizes and model shapes.
+CONFIG4_N = 512
+CONFIG4_MAX_DENSE = 150  # dense lightcurves per asteroid (the reference allows fewer than 200 in all)
+
+
+def config4(i, **kw):
+    """Asteroid i (0 <= i < 512) of the batched workload: ndata log-uniform in 100-20000 (5/6 of it in
+    dense relative lightcurves of 50 points, at most 150 of them, the rest sparse calibrated points),
+    Lmax = Mmax in {3, 4, 6}, nrows in {6, 8}, a random true period in 2.5-18 h, and a scan window of
+    1000-20000 trial periods (log-uniform) around it.  The convergence stop (rms improvement below
+    0.001, 10-100 iterations) gives the fits uneven iteration counts.  Seeds 1004 (the shape of the
+    batch) and 40000 + i (the asteroid)."""
+    rng = np.random.default_rng([1004, i])
+    ndata = int(round(np.exp(rng.uniform(np.log(100), np.log(19997)))))
+    n_dense = min(CONFIG4_MAX_DENSE, int(round(ndata * 5 / 6 / 50)))
+    n_sparse = ndata - 50 * n_dense
+    lmax = int(rng.choice([3, 4, 6]))
+    nrows = int(rng.choice([6, 8]))
+    period = float(rng.uniform(2.5, 18.0))
+    nper = int(round(np.exp(rng.uniform(np.log(1000), np.log(20000)))))
+    span_years = 15.0
+    # trial frequencies f0 - (n - 1) df with df = 0.5 / (span_days * 24) (period_search_BOINC.cpp:784-786)
+    df = 0.5 / (span_years * 365.25 * 24)
+    f_true = 1 / period
+    per_start = 1 / (f_true + 0.5 * nper * df)
+    per_end = 1 / (f_true - 0.5 * nper * df)
+    base = dict(seed=40000 + i, n_dense=n_dense, dense_points=50, n_sparse=n_sparse, span_years=span_years,
+                period_h=period, per_start=round(per_start, 6), per_end=round(per_end, 6), fit_lmax=lmax,
+                fit_nrows=nrows, stop_condition=0.001, n_iter_min=10)
+    base.update(kw)
+    return SynthSpec(**base)
