@@ -140,12 +140,30 @@ int ps_get_info(const ps_handle *h, ps_info *info);
 int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r, ps_fit_result *out);
 /* The fits of several prepared problems (asteroids) on one device as one batch: job j fits frequency
    freq[j] (1/h) with pole pole[j] (1..10) of problem hs[prob[j]] at convexity weight
-   conw_r[prob[j]], into out[j].  Each problem's jobs run as one launch on its own stream; all
-   launches are issued, largest problem first, before any result is waited for, so problems too small
-   to fill the device run concurrently.  Results equal ps_fit_batch's for the same fits.  Each problem
-   handle may appear once.  The per-job body is that of period_search_BOINC.cpp:817-921. */
+   conw_r[prob[j]], into out[j].  When every problem runs the FP64 J-free kernels this is
+   ps_batch_create + ps_batch_run + ps_batch_release (one persistent launch); otherwise each problem's
+   jobs run as one launch on its own stream, all issued (largest problem first) before any result is
+   waited for.  Results equal ps_fit_batch's for the same fits.  Each problem handle may appear once.
+   The per-job body is that of period_search_BOINC.cpp:817-921. */
 int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, int njobs, const int *prob, const double *freq,
                  const int *pole, ps_fit_result *out);
+
+/* A batch of many problems (asteroids) on one device, for repeated runs.  The reference runs one
+   asteroid per process and has no counterpart; the per-job body is period_search_BOINC.cpp:817-921.
+   ps_batch_create: the problem table on the device and one pool of per-workgroup buffers sized for the
+   largest problem (the problems must run the FP64 J-free kernels: precision PS_FP64 and no relative
+   lightcurve longer than 128 points; else PS_ERR_UNSUPPORTED).  The handles must outlive the batch.
+   ps_batch_run: njobs fits (job j: problem prob[j], frequency freq[j] in 1/h, pole pole[j] in 1..10, at
+   convexity weight conw_r[prob[j]]) as one graph launch: a queue reset, then persistent workgroups
+   that claim jobs from a device work queue, longest first, until it is empty.  out[j] equals
+   ps_fit_batch's result for the same fit.  The graph is kept and replayed while the job counts repeat.
+   ps_batch_kernel_time: device time of the graph launches since the last reset (HIP events). */
+typedef struct ps_batch ps_batch;
+int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out);
+int ps_batch_run(ps_batch *b, const double *conw_r, int njobs, const int *prob, const double *freq, const int *pole,
+                 ps_fit_result *out);
+int ps_batch_kernel_time(ps_batch *b, double *ms, int *launches, int reset);
+void ps_batch_release(ps_batch *b);
 /* One round of the convexity-weight search over the first min(10, nmax) periods; ave_dark = mean
    best-pole dark-facet % (NaN counted as 1.0). */
 int ps_conw_round(ps_handle *h, double conw_r, double *ave_dark);

@@ -5,7 +5,7 @@ include/periodsearch_amd.h) and the period_search_amd executable; this package h
 front end (api), the synthetic-input generator (synth) and the build helper.
 """
 from ._lib import EXE_PATH, LIB_PATH, N_POLES, PS_FP64, PS_MIXED, PsError, lib  # noqa: F401
-from .api import FIT_DTYPE, LINE_DTYPE, PeriodSearch, fit_multi, format_line, run_files  # noqa: F401
+from .api import FIT_DTYPE, LINE_DTYPE, Batch, PeriodSearch, fit_multi, format_line, run_files  # noqa: F401
 
-__all__ = ["PeriodSearch", "fit_multi", "format_line", "run_files", "PS_FP64", "PS_MIXED", "N_POLES", "PsError", "lib",
+__all__ = ["PeriodSearch", "Batch", "fit_multi", "format_line", "run_files", "PS_FP64", "PS_MIXED", "N_POLES", "PsError", "lib",
            "LIB_PATH", "EXE_PATH", "LINE_DTYPE", "FIT_DTYPE"]

@@ -83,6 +83,10 @@ SIGNATURES = {
     "ps_get_info": (C.c_int, [C.c_void_p, C.POINTER(PsInfo)]),
     "ps_fit_batch": (C.c_int, [C.c_void_p, C.c_int, DP, IP, C.c_double, C.POINTER(PsFitResult)]),
     "ps_fit_multi": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, DP, C.c_int, IP, DP, IP, C.POINTER(PsFitResult)]),
+    "ps_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "ps_batch_run": (C.c_int, [C.c_void_p, DP, C.c_int, IP, DP, IP, C.POINTER(PsFitResult)]),
+    "ps_batch_kernel_time": (C.c_int, [C.c_void_p, DP, IP, C.c_int]),
+    "ps_batch_release": (None, [C.c_void_p]),
     "ps_conw_round": (C.c_int, [C.c_void_p, C.c_double, DP]),
     "ps_conw_search": (C.c_int, [C.c_void_p, DP, IP]),
     "ps_scan": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(PsLine)]),

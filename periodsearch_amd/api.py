@@ -218,3 +218,56 @@ def fit_multi(searches, conw_r, prob, freq, pole):
     L.check(lib.ps_fit_multi(hs, len(searches), _dp(cw), len(freq), prob.ctypes.data_as(L.IP), _dp(freq),
                              pole.ctypes.data_as(L.IP), out.ctypes.data_as(C.POINTER(L.PsFitResult))), "ps_fit_multi")
     return out
+
+
+class Batch:
+    """Many prepared problems (asteroids) fitted as one persistent launch over a device work queue
+    (ps_batch_create / ps_batch_run): job j fits freq[j] with pole pole[j] of searches[prob[j]] at
+    convexity weight conw_r[prob[j]].  The searches must stay open while the batch is used."""
+
+    def __init__(self, searches):
+        self._searches = list(searches)
+        if not self._searches:
+            raise ValueError("Batch: no problems")
+        self._lib = self._searches[0]._lib
+        ptrs = [s._handle() for s in self._searches]
+        hs = (C.c_void_p * len(ptrs))(*[p.value if isinstance(p, C.c_void_p) else p for p in ptrs])
+        b = C.c_void_p()
+        L.check(self._lib.ps_batch_create(hs, len(ptrs), C.byref(b)), "ps_batch_create")
+        self._b = b
+
+    def run(self, conw_r, prob, freq, pole):
+        cw = np.ascontiguousarray(conw_r, dtype=np.float64)
+        prob = np.ascontiguousarray(prob, dtype=np.int32)
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        pole = np.ascontiguousarray(pole, dtype=np.int32)
+        if not (len(cw) == len(self._searches) and len(prob) == len(freq) == len(pole)):
+            raise ValueError("Batch.run: inconsistent sizes")
+        out = np.zeros(len(freq), dtype=FIT_DTYPE)
+        L.check(self._lib.ps_batch_run(self._b, _dp(cw), len(freq), prob.ctypes.data_as(L.IP), _dp(freq),
+                                       pole.ctypes.data_as(L.IP), out.ctypes.data_as(C.POINTER(L.PsFitResult))),
+                "ps_batch_run")
+        return out
+
+    def kernel_time(self, reset=False):
+        ms = C.c_double()
+        n = C.c_int()
+        L.check(self._lib.ps_batch_kernel_time(self._b, C.byref(ms), C.byref(n), int(reset)), "ps_batch_kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        if self._b is not None:
+            self._lib.ps_batch_release(self._b)
+            self._b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

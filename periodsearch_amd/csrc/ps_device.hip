@@ -11,15 +11,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "../../include/periodsearch_amd.h"
 #include "ps_internal.h"
 #include "ps_kernels.h"
+#include "ps_fit2.h"
 
 using psk::DevProblem;
 using psk::Slots;
+using psk::Slots2;
 
 namespace {
 constexpr int kWG = psk::kWG;
@@ -55,6 +58,83 @@ Variant pick_variant(int nc, int nf) {
     v.lds_mix = nf * (16 * v.nt + v.r) <= 2 * psk::kDsphLds;
     return v;
 }
+
+// Pass schedule of the J-free kernels: the observed points and the three convexity rows (index
+// nobs .. nobs+2) placed into passes of at most kPassRows positions (kPassTiles tiles of 16), every
+// relative lightcurve whole inside one pass.  Relative lightcurves are packed first, in order; the
+// calibrated points then fill each pass up to its last started tile, then whole tiles of the passes
+// with room, then new passes; the convexity rows go to the end of the last pass.  perm[q] = the
+// point at position q (-1: padding).  Returns false when a relative lightcurve is longer than a pass.
+struct Schedule {
+    std::vector<int> perm, pass_tile0, pass_unit0, unit_rows, unit_lc;
+    int conv_q = 0;
+};
+bool build_schedule(const std::vector<int> &lc_start, const std::vector<int> &lc_inrel, int nlc_obs, Schedule &s) {
+    constexpr int cap = psk::kPassRows;
+    std::vector<std::vector<int>> pass;   // positions of each pass (point indices)
+    std::vector<std::vector<int>> units;  // per pass: relative lightcurves
+    for (int L = 0; L < nlc_obs; ++L) {
+        if (!lc_inrel[L]) continue;
+        const int n = lc_start[L + 1] - lc_start[L];
+        if (n > cap) return false;
+        if (pass.empty() || int(pass.back().size()) + n > cap) {
+            pass.emplace_back();
+            units.emplace_back();
+        }
+        units.back().push_back(L);
+        for (int p = lc_start[L]; p < lc_start[L + 1]; ++p) pass.back().push_back(p);
+    }
+    std::vector<int> cal;
+    for (int L = 0; L < nlc_obs; ++L)
+        if (!lc_inrel[L])
+            for (int p = lc_start[L]; p < lc_start[L + 1]; ++p) cal.push_back(p);
+    size_t next = 0;
+    auto room = [&](size_t k, int upto) { return std::max(0, upto - int(pass[k].size())); };
+    for (size_t k = 0; k < pass.size() && next < cal.size(); ++k) {  // to the end of the started tile
+        int n = room(k, (int(pass[k].size()) + 15) / 16 * 16);
+        while (n-- > 0 && next < cal.size()) pass[k].push_back(cal[next++]);
+    }
+    for (size_t k = 0; k < pass.size() && next < cal.size(); ++k) {  // whole tiles
+        int n = room(k, cap);
+        while (n-- > 0 && next < cal.size()) pass[k].push_back(cal[next++]);
+    }
+    while (next < cal.size()) {
+        pass.emplace_back();
+        units.emplace_back();
+        while (int(pass.back().size()) < cap && next < cal.size()) pass.back().push_back(cal[next++]);
+    }
+    const int nobs = lc_start[nlc_obs];
+    if (pass.empty() || int(pass.back().size()) + 3 > cap) {
+        pass.emplace_back();
+        units.emplace_back();
+    }
+    for (int c = 0; c < 3; ++c) pass.back().push_back(nobs + c);
+    s.perm.clear();
+    s.pass_tile0.assign(1, 0);
+    s.pass_unit0.assign(1, 0);
+    s.unit_rows.clear();
+    s.unit_lc.clear();
+    for (size_t k = 0; k < pass.size(); ++k) {
+        const int base = int(s.perm.size());
+        // the relative lightcurves were placed first, in order, so each is a run from its first point
+        for (int L : units[k]) {
+            int r0 = -1;
+            for (int i = 0; i < int(pass[k].size()); ++i)
+                if (pass[k][i] == lc_start[L]) r0 = i;
+            s.unit_rows.push_back(r0 | ((r0 + lc_start[L + 1] - lc_start[L]) << 8));
+            s.unit_lc.push_back(L);
+        }
+        for (int p : pass[k]) {
+            if (p == nobs) s.conv_q = int(s.perm.size());
+            s.perm.push_back(p);
+        }
+        const int ntile = (int(pass[k].size()) + 15) / 16;
+        s.perm.resize(size_t(base) + 16 * ntile, -1);
+        s.pass_tile0.push_back(s.pass_tile0.back() + ntile);
+        s.pass_unit0.push_back(int(s.unit_lc.size()));
+    }
+    return true;
+}
 }  // namespace
 
 struct ps_handle {
@@ -70,6 +150,16 @@ struct ps_handle {
     Slots S{};
     Variant var{};
     double *d_wq = nullptr;
+    // the J-free kernels (ps_fit2.h): FP64 fits and evaluations when every relative lightcurve fits
+    // one pass; P2 holds the points in pass-schedule order
+    bool v2 = false;
+    DevProblem P2{};
+    const DevProblem *dP2 = nullptr;
+    Slots2 S2{};
+    int slots2 = 0;
+    int nseg = 0;
+    std::vector<double> wq2_host;
+    double *d_wq2 = nullptr;
     std::vector<void *> owned;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -122,12 +212,51 @@ int dispatch(const Variant &v, bool mix, F &&f) {
     return PS_ERR_UNSUPPORTED;
 }
 
+// Calls f.template operator()<NT, R>() for the coefficient layout of the J-free kernels.
+template <class F>
+int dispatch2(const Variant &v, F &&f) {
+#define PS_VARIANT2(NT_, R_) \
+    if (v.nt == NT_ && v.r == R_) return f.template operator()<NT_, R_>();
+    PS_VARIANT2(1, 0)
+    PS_VARIANT2(1, 9)
+    PS_VARIANT2(2, 4)
+    PS_VARIANT2(3, 1)
+    PS_VARIANT2(4, 0)
+#undef PS_VARIANT2
+    return PS_ERR_UNSUPPORTED;
+}
+
+// The queue kernel's index of a coefficient layout (the order of its switch).
+int layout_id(const Variant &v) {
+    if (v.nt == 1 && v.r == 0) return 0;
+    if (v.nt == 1 && v.r == 9) return 1;
+    if (v.nt == 2 && v.r == 4) return 2;
+    if (v.nt == 3 && v.r == 1) return 3;
+    return 4;
+}
+
+template <int NT, int R, bool LS>
+int launch_fit2(ps_handle *h, int grid, int nfits, double conw_r) {
+    hipLaunchKernelGGL((psk::fit_kernel2<NT, R, LS, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream, h->dP2, h->S2,
+                       h->d_freq, h->d_pole, nfits, conw_r, h->d_out);
+    return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
+}
+
+// The J-free kernels run the FP64 fits and evaluations of a problem whose schedule they accept.
+bool use_v2(const ps_handle *h) { return h->v2 && h->precision == PS_FP64; }
+
 int set_conw(ps_handle *h, double conw_r) {
     if (h->conw_loaded == conw_r) return PS_OK;
     const double sig = 1 / conw_r;  // period_search_BOINC.cpp:721-726
     for (int j = 0; j < 3; ++j) h->wq_host[h->ndata + j] = 1 / (sig * sig);
     HIPCHECK(hipMemcpyAsync(h->d_wq + h->ndata, h->wq_host.data() + h->ndata, 3 * sizeof(double),
                             hipMemcpyHostToDevice, h->stream));
+    if (h->v2) {
+        const int q = h->P2.conv_q;
+        for (int j = 0; j < 3; ++j) h->wq2_host[q + j] = 1 / (sig * sig);
+        HIPCHECK(hipMemcpyAsync(h->d_wq2 + q, h->wq2_host.data() + q, 3 * sizeof(double), hipMemcpyHostToDevice,
+                                h->stream));
+    }
     h->conw_loaded = conw_r;
     return PS_OK;
 }
@@ -340,7 +469,8 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     P.darea = dev_upload(geo->darea, nf, o);
     h->var = pick_variant(ncoef, nf);
     P.dpitch = 16 * h->var.nt + h->var.r;
-    std::vector<double> dsph_pad(size_t(nf) * P.dpitch, 0.0);
+    // four zero facets past the end: the J-free facet loop loads the next chunk of Dsph unconditionally
+    std::vector<double> dsph_pad(size_t(nf + 4) * P.dpitch, 0.0);
     for (int i = 0; i < nf; ++i)
         for (int k = 0; k < ncoef; ++k) dsph_pad[size_t(i) * P.dpitch + k] = geo->dsph[size_t(i) * ncoef + k];
     P.dsph = dev_upload(dsph_pad.data(), dsph_pad.size(), o);
@@ -359,7 +489,75 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     h->dP = dev_upload(&P, 1, o);
     if (!h->dP) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
 
-    // resident workgroups: one scratch slot each
+    // J-free kernels: the points in pass-schedule order (padding positions all zero, the arrays one tile
+    // longer than the schedule for the second tile a wave reads past the end of an odd pass)
+    // PSAMD_JFREE=0 in the environment keeps the materialised-Jacobian kernels (A/B comparisons)
+    const char *jf = std::getenv("PSAMD_JFREE");
+    const bool allow_v2 = !(jf && std::strcmp(jf, "0") == 0);
+    Schedule sc;
+    if (allow_v2 && mfit + 1 <= psk::kJP && nf <= kMaxFac && build_schedule(lc_start, lc_inrel, in->lcurves, sc)) {
+        // the arrays run a whole pass past the schedule: a wave reads its second tile, and the per-row
+        // loads of the last pass read 128 rows, whatever the pass holds
+        const int nsched = int(sc.perm.size()), nq = nsched + psk::kPassRows;
+        std::vector<double> qt(nq, 0), qex(nq, 0), qey(nq, 0), qez(nq, 0), qsx(nq, 0), qsy(nq, 0), qsz(nq, 0),
+            qal(nq, 0), qy(nq, 0), qsig(nq, 1), qsl(nq, 0);
+        std::vector<int> qrel(nq, 0), qperm(nq, -1);
+        h->wq2_host.assign(nq, 0.0);
+        for (int q = 0; q < nsched; ++q) {
+            const int p = sc.perm[q];
+            if (p < 0 || p >= nobs) continue;  // padding; the convexity rows keep zeros (their weight: set_conw)
+            qt[q] = t[p];
+            qex[q] = ex[p];
+            qey[q] = ey[p];
+            qez[q] = ez[p];
+            qsx[q] = sx[p];
+            qsy[q] = sy[p];
+            qsz[q] = sz[p];
+            qal[q] = alpha[p];
+            qy[q] = y[p];
+            qsig[q] = sig[p];
+            h->wq2_host[q] = h->wq_host[p];
+            qrel[q] = lc_inrel[lc_of[p]];
+            qperm[q] = p;
+            if (qrel[q]) qsl[q] = sig[p] * (lc_start[lc_of[p] + 1] - lc_start[lc_of[p]]);  // mrqcof.c:94
+        }
+        std::vector<int> fpos(ma, -1);
+        for (int j = 0; j < mfit; ++j) fpos[fit_idx[j]] = j;
+        DevProblem &Q = h->P2;
+        Q = P;
+        Q.t = dev_upload(qt.data(), nq, o);
+        Q.ex = dev_upload(qex.data(), nq, o);
+        Q.ey = dev_upload(qey.data(), nq, o);
+        Q.ez = dev_upload(qez.data(), nq, o);
+        Q.sx = dev_upload(qsx.data(), nq, o);
+        Q.sy = dev_upload(qsy.data(), nq, o);
+        Q.sz = dev_upload(qsz.data(), nq, o);
+        Q.alpha = dev_upload(qal.data(), nq, o);
+        Q.y = dev_upload(qy.data(), nq, o);
+        Q.sig = dev_upload(qsig.data(), nq, o);
+        h->d_wq2 = dev_upload(h->wq2_host.data(), nq, o);
+        Q.wq = h->d_wq2;
+        Q.npass = int(sc.pass_tile0.size()) - 1;
+        Q.nsched = nsched;
+        Q.conv_q = sc.conv_q;
+        Q.c00col = fpos[0];
+        Q.pass_tile0 = dev_upload(sc.pass_tile0.data(), sc.pass_tile0.size(), o);
+        Q.pass_unit0 = dev_upload(sc.pass_unit0.data(), sc.pass_unit0.size(), o);
+        Q.unit_rows = dev_upload(sc.unit_rows.data(), sc.unit_rows.size(), o);
+        Q.unit_lc = dev_upload(sc.unit_lc.data(), sc.unit_lc.size(), o);
+        Q.qrel = dev_upload(qrel.data(), nq, o);
+        Q.qperm = dev_upload(qperm.data(), nq, o);
+        Q.fpos = dev_upload(fpos.data(), fpos.size(), o);
+        Q.qsl = dev_upload(qsl.data(), nq, o);
+        for (void *ptr : o)
+            if (!ptr) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
+        h->dP2 = dev_upload(&Q, 1, o);
+        if (!h->dP2) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
+        h->v2 = true;
+    }
+
+    // resident workgroups: one scratch slot each; the slots are allocated on first use (a handle of a
+    // multi-problem batch may never run a launch of its own)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(ps_fail(PS_ERR_DEVICE, "device query failed"));
     int per_cu = 1;
@@ -368,27 +566,55 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
         return 0;
     });
     h->slots = std::max(1, per_cu) * prop.multiProcessorCount;
-    Slots &S = h->S;
-    S.j_stride = size_t(npts) * psk::kMap;
-    S.y_stride = size_t(npts);
-    S.dave_stride = size_t(nseg) * psk::kMap;  // per-segment partial column sums
-    S.a_stride = size_t(mfit) * mfit;
-    S.b_stride = size_t(mfit);
-    const size_t ns = size_t(h->slots);
-    if (hipMalloc(&S.J, sizeof(double) * S.j_stride * ns) != hipSuccess ||
-        hipMalloc(&S.ytemp, sizeof(double) * S.y_stride * ns) != hipSuccess ||
-        hipMalloc(&S.dave, sizeof(double) * S.dave_stride * ns) != hipSuccess ||
-        hipMalloc(&S.alpha, sizeof(double) * S.a_stride * 2 * ns) != hipSuccess ||
-        hipMalloc(&S.beta, sizeof(double) * S.b_stride * 2 * ns) != hipSuccess)
-        return fail(ps_fail(PS_ERR_NOMEM, "scratch allocation failed (%d slots)", h->slots));
-    o.push_back(S.J);
-    o.push_back(S.ytemp);
-    o.push_back(S.dave);
-    o.push_back(S.alpha);
-    o.push_back(S.beta);
+    h->nseg = nseg;
+    if (h->v2) {
+        int per_cu2 = 1;
+        dispatch2(h->var, [&]<int NT, int R>() {
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, psk::fit_kernel2<NT, R, false, kMaxFac>, psk::kWG2, 0);
+            return 0;
+        });
+        h->slots2 = std::max(1, per_cu2) * prop.multiProcessorCount;
+    }
     *out = h;
     return PS_OK;
 }
+
+namespace {
+// Allocates the per-slot buffers of the kernels the handle is about to launch (once).
+int ensure_scratch(ps_handle *h, bool v2) {
+    auto alloc = [&](double **p, size_t n) {
+        if (hipMalloc(p, sizeof(double) * std::max<size_t>(n, 1)) != hipSuccess) return false;
+        h->owned.push_back(*p);
+        return true;
+    };
+    if (v2 && !h->S2.alpha) {
+        Slots2 T{};
+        T.a_stride = size_t(h->mfit) * h->mfit;
+        T.b_stride = size_t(h->mfit);
+        T.f_stride = size_t(h->nfac);
+        const size_t n = size_t(h->slots2);
+        if (!alloc(&T.alpha, T.a_stride * 2 * n) || !alloc(&T.beta, T.b_stride * 2 * n) ||
+            !alloc(&T.stash, psk::kStashPerSlot * n) || !alloc(&T.area, T.f_stride * n))
+            return ps_fail(PS_ERR_NOMEM, "scratch allocation failed (%d slots)", h->slots2);
+        h->S2 = T;
+    }
+    if (!v2 && !h->S.J) {
+        Slots S{};
+        S.j_stride = size_t(h->npts) * psk::kMap;
+        S.y_stride = size_t(h->npts);
+        S.dave_stride = size_t(h->nseg) * psk::kMap;  // per-segment partial column sums
+        S.a_stride = size_t(h->mfit) * h->mfit;
+        S.b_stride = size_t(h->mfit);
+        const size_t n = size_t(h->slots);
+        if (!alloc(&S.ytemp, S.y_stride * n) || !alloc(&S.dave, S.dave_stride * n) ||
+            !alloc(&S.alpha, S.a_stride * 2 * n) || !alloc(&S.beta, S.b_stride * 2 * n) ||
+            !alloc(&S.J, S.j_stride * n))
+            return ps_fail(PS_ERR_NOMEM, "scratch allocation failed (%d slots)", h->slots);
+        h->S = S;
+    }
+    return PS_OK;
+}
+}  // namespace
 
 extern "C" int ps_set_precision(ps_handle *h, int precision) {
     if (!h) return ps_fail(PS_ERR_ARG, "ps_set_precision: null handle");
@@ -413,7 +639,7 @@ extern "C" int ps_get_info(const ps_handle *h, ps_info *info) {
     info->freq_step = h->freq_step;
     info->escl = h->escl;
     info->conw_r0 = h->conw_r0;
-    info->slots = h->slots;
+    info->slots = use_v2(h) ? h->slots2 : h->slots;
     info->precision = h->precision;
     return PS_OK;
 }
@@ -436,15 +662,23 @@ int enqueue_fits(ps_handle *h, int nfits, const double *freq, const int *pole, d
         h->cap_fits = nfits;
     }
     int rc = set_conw(h, conw_r);
+    if (rc == PS_OK) rc = ensure_scratch(h, use_v2(h));
     if (rc != PS_OK) return rc;
     HIPCHECK(hipMemcpyAsync(h->d_freq, freq, sizeof(double) * nfits, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(hipMemcpyAsync(h->d_pole, pole, sizeof(int) * nfits, hipMemcpyHostToDevice, h->stream));
-    const int grid = std::min(nfits, h->slots);
     HIPCHECK(hipEventRecord(h->ev0, h->stream));
-    rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
-        return h->ls ? launch_fit<NT, R, LDSB, MIX, true>(h, grid, nfits, conw_r)
-                     : launch_fit<NT, R, LDSB, MIX, false>(h, grid, nfits, conw_r);
-    });
+    if (use_v2(h)) {
+        const int grid = std::min(nfits, h->slots2);
+        rc = dispatch2(h->var, [&]<int NT, int R>() {
+            return h->ls ? launch_fit2<NT, R, true>(h, grid, nfits, conw_r) : launch_fit2<NT, R, false>(h, grid, nfits, conw_r);
+        });
+    } else {
+        const int grid = std::min(nfits, h->slots);
+        rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
+            return h->ls ? launch_fit<NT, R, LDSB, MIX, true>(h, grid, nfits, conw_r)
+                         : launch_fit<NT, R, LDSB, MIX, false>(h, grid, nfits, conw_r);
+        });
+    }
     if (rc != PS_OK) return ps_fail(rc, "fit kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIPCHECK(hipEventRecord(h->ev1, h->stream));
     return PS_OK;
@@ -498,6 +732,16 @@ extern "C" int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, 
     if (rc != PS_OK) return rc;
     if (njobs == 0) return PS_OK;
     HIPCHECK(hipSetDevice(hs[0]->device));
+    // FP64 problems of the J-free kernels: one persistent launch over a device work queue
+    bool all_v2 = true;
+    for (int i = 0; i < nh; ++i) all_v2 = all_v2 && use_v2(hs[i]);
+    if (all_v2) {
+        ps_batch *b = nullptr;
+        rc = ps_batch_create(hs, nh, &b);
+        if (rc == PS_OK) rc = ps_batch_run(b, conw_r, njobs, prob, freq, pole, out);
+        ps_batch_release(b);
+        return rc;
+    }
     // the jobs of each problem, and the problems in order of decreasing work (longest first)
     std::vector<std::vector<int>> jobs(nh);
     for (int j = 0; j < njobs; ++j) jobs[prob[j]].push_back(j);
@@ -529,6 +773,206 @@ extern "C" int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, 
     return PS_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Batches of many problems: one persistent launch draining a device work queue, captured in a graph.
+// ---------------------------------------------------------------------------------------------
+struct ps_batch {
+    int device = 0;
+    std::vector<ps_handle *> hs;
+    std::vector<double> cost;  // estimated work of one fit of each problem
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> owned;
+    const DevProblem **d_probs = nullptr;
+    int *d_layout = nullptr;
+    double *d_conw = nullptr;
+    int *d_counter = nullptr;
+    Slots2 pool{};
+    int slots = 0;
+    psk::QueueJob *d_jobs = nullptr;
+    ps_fit_result *d_out = nullptr;
+    int cap_jobs = 0;
+    hipGraphExec_t exec = nullptr;
+    int exec_n0 = -1, exec_n1 = -1;
+    double kernel_ms = 0;
+    int launches = 0;
+};
+
+extern "C" void ps_batch_release(ps_batch *b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    for (void *p : b->owned) hipFree(p);
+    hipFree(b->d_jobs);
+    hipFree(b->d_out);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out) {
+    if (!hs || nh < 1 || !out) return ps_fail(PS_ERR_ARG, "ps_batch_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < nh; ++i) {
+        if (!hs[i]) return ps_fail(PS_ERR_ARG, "ps_batch_create: problem %d is null", i);
+        if (hs[i]->device != hs[0]->device) return ps_fail(PS_ERR_ARG, "ps_batch_create: problems on different devices");
+        if (!use_v2(hs[i]))
+            return ps_fail(PS_ERR_UNSUPPORTED, "ps_batch_create: problem %d needs the FP64 J-free kernels (precision FP64, "
+                                               "every relative lightcurve <= %d points)", i, psk::kPassRows);
+    }
+    HIPCHECK(hipSetDevice(hs[0]->device));
+    auto *b = new ps_batch();
+    b->device = hs[0]->device;
+    b->hs.assign(hs, hs + nh);
+    auto fail = [&](int rc) {
+        ps_batch_release(b);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+        return fail(ps_fail(PS_ERR_DEVICE, "stream/event creation failed"));
+    std::vector<const DevProblem *> probs(nh);
+    std::vector<int> layout(nh);
+    size_t amax = 1, bmax = 1, fmax = 1;
+    b->cost.resize(nh);
+    for (int i = 0; i < nh; ++i) {
+        const ps_handle *h = hs[i];
+        probs[i] = h->dP2;
+        layout[i] = layout_id(h->var);
+        amax = std::max(amax, size_t(h->mfit) * h->mfit);
+        bmax = std::max(bmax, size_t(h->mfit));
+        fmax = std::max(fmax, size_t(h->nfac));
+        const double iters = h->P2.n_iter_max > 0 ? h->P2.n_iter_max : h->P2.n_iter_min;
+        b->cost[i] = (iters + 1) * 16.0 * h->P2.pass_tile0[h->P2.npass] *
+                     (2.0 * h->nfac * h->ncoef + double(h->mfit) * (h->mfit + 1));
+    }
+    auto &o = b->owned;
+    b->d_probs = dev_upload(probs.data(), nh, o);
+    b->d_layout = dev_upload(layout.data(), nh, o);
+    std::vector<double> zero(nh, 0.0);
+    b->d_conw = dev_upload(zero.data(), nh, o);
+    int z2[2] = {0, 0};
+    b->d_counter = dev_upload(z2, 2, o);
+    for (void *p : o)
+        if (!p) return fail(ps_fail(PS_ERR_DEVICE, "device upload failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) return fail(ps_fail(PS_ERR_DEVICE, "device query failed"));
+    int per_cu = 1;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::queue_kernel2<false>, psk::kWG2, 0);
+    b->slots = std::max(1, per_cu) * prop.multiProcessorCount;
+    Slots2 &T = b->pool;
+    T.a_stride = amax;
+    T.b_stride = bmax;
+    T.f_stride = fmax;
+    const size_t n = size_t(b->slots);
+    double *pa = nullptr, *pb = nullptr, *ps = nullptr, *pf = nullptr;
+    if (hipMalloc(&pa, sizeof(double) * amax * 2 * n) != hipSuccess || (o.push_back(pa), false) ||
+        hipMalloc(&pb, sizeof(double) * bmax * 2 * n) != hipSuccess || (o.push_back(pb), false) ||
+        hipMalloc(&ps, sizeof(double) * psk::kStashPerSlot * n) != hipSuccess || (o.push_back(ps), false) ||
+        hipMalloc(&pf, sizeof(double) * fmax * n) != hipSuccess || (o.push_back(pf), false))
+        return fail(ps_fail(PS_ERR_NOMEM, "batch scratch allocation failed (%d slots)", b->slots));
+    T.alpha = pa;
+    T.beta = pb;
+    T.stash = ps;
+    T.area = pf;
+    *out = b;
+    return PS_OK;
+}
+
+extern "C" int ps_batch_run(ps_batch *b, const double *conw_r, int njobs, const int *prob, const double *freq,
+                            const int *pole, ps_fit_result *out) {
+    if (!b || !conw_r || njobs < 0 || (njobs && (!prob || !freq || !pole || !out)))
+        return ps_fail(PS_ERR_ARG, "ps_batch_run: bad argument");
+    const int nh = int(b->hs.size());
+    for (int i = 0; i < nh; ++i)
+        if (!(conw_r[i] > 0)) return ps_fail(PS_ERR_ARG, "ps_batch_run: conw_r[%d] must be positive", i);
+    for (int j = 0; j < njobs; ++j)
+        if (prob[j] < 0 || prob[j] >= nh) return ps_fail(PS_ERR_ARG, "ps_batch_run: job %d has problem %d", j, prob[j]);
+    int rc = check_fits("ps_batch_run", njobs, freq, pole);
+    if (rc != PS_OK) return rc;
+    if (njobs == 0) return PS_OK;
+    HIPCHECK(hipSetDevice(b->device));
+    // the convexity rows' weight of every problem (each problem's own stream), then the job queue:
+    // problems without the Lambert column first, then those with it; each group longest first
+    for (int i = 0; i < nh; ++i) {
+        rc = set_conw(b->hs[i], conw_r[i]);
+        if (rc != PS_OK) return rc;
+    }
+    for (int i = 0; i < nh; ++i) HIPCHECK(hipStreamSynchronize(b->hs[i]->stream));
+    std::vector<int> order(njobs);
+    for (int j = 0; j < njobs; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const bool lx = b->hs[prob[x]]->ls, ly = b->hs[prob[y]]->ls;
+        if (lx != ly) return ly;
+        return b->cost[prob[x]] > b->cost[prob[y]];
+    });
+    std::vector<psk::QueueJob> jobs(njobs);
+    int n0 = 0;
+    for (int k = 0; k < njobs; ++k) {
+        const int j = order[k];
+        jobs[k] = {freq[j], prob[j], pole[j], j, 0};
+        n0 += !b->hs[prob[j]]->ls;
+    }
+    const int n1 = njobs - n0;
+    if (njobs > b->cap_jobs) {
+        if (b->exec) hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        (void)hipFree(b->d_jobs);
+        (void)hipFree(b->d_out);
+        b->d_jobs = nullptr;
+        b->d_out = nullptr;
+        b->cap_jobs = 0;
+        HIPCHECK(hipMalloc(&b->d_jobs, sizeof(psk::QueueJob) * njobs));
+        HIPCHECK(hipMalloc(&b->d_out, sizeof(ps_fit_result) * njobs));
+        b->cap_jobs = njobs;
+    }
+    HIPCHECK(hipMemcpyAsync(b->d_conw, conw_r, sizeof(double) * nh, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(psk::QueueJob) * njobs, hipMemcpyHostToDevice, b->stream));
+    if (!b->exec || b->exec_n0 != n0 || b->exec_n1 != n1) {
+        if (b->exec) hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(psk::queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_counter, 2);
+        if (n0)
+            hipLaunchKernelGGL(psk::queue_kernel2<false>, dim3(std::min(n0, b->slots)), dim3(psk::kWG2), 0, b->stream,
+                               b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs, n0, b->d_counter, b->d_out);
+        if (n1)
+            hipLaunchKernelGGL(psk::queue_kernel2<true>, dim3(std::min(n1, b->slots)), dim3(psk::kWG2), 0, b->stream,
+                               b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs + n0, n1, b->d_counter + 1,
+                               b->d_out);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (e != hipSuccess) return ps_fail(PS_ERR_DEVICE, "ps_batch_run: graph capture failed: %s", hipGetErrorString(e));
+        b->exec_n0 = n0;
+        b->exec_n1 = n1;
+    }
+    HIPCHECK(hipEventRecord(b->ev0, b->stream));
+    HIPCHECK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHECK(hipEventRecord(b->ev1, b->stream));
+    HIPCHECK(hipMemcpyAsync(out, b->d_out, sizeof(ps_fit_result) * njobs, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->kernel_ms += ms;
+    b->launches += 1;
+    return PS_OK;
+}
+
+extern "C" int ps_batch_kernel_time(ps_batch *b, double *ms, int *launches, int reset) {
+    if (!b) return ps_fail(PS_ERR_ARG, "ps_batch_kernel_time: null batch");
+    if (ms) *ms = b->kernel_ms;
+    if (launches) *launches = b->launches;
+    if (reset) {
+        b->kernel_ms = 0;
+        b->launches = 0;
+    }
+    return PS_OK;
+}
+
 extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, double *alpha, double *beta,
                        double *chisq, double *ymod, double *dyda) {
     if (!h || nvec < 0 || (nvec && !a)) return ps_fail(PS_ERR_ARG, "ps_eval: bad argument");
@@ -537,6 +981,7 @@ extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, d
     if (!(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_eval: conw_r must be positive");
     HIPCHECK(hipSetDevice(h->device));
     int rc = set_conw(h, conw_r);
+    if (rc == PS_OK) rc = ensure_scratch(h, use_v2(h));
     if (rc != PS_OK) return rc;
     const size_t mf2 = size_t(h->mfit) * h->mfit;
     double *d_a = nullptr, *d_al = nullptr, *d_be = nullptr, *d_ch = nullptr, *d_y = nullptr, *d_dy = nullptr;
@@ -559,11 +1004,19 @@ extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, d
         return ps_fail(PS_ERR_NOMEM, "ps_eval: device allocation failed");
     }
     hipError_t e = hipMemcpyAsync(d_a, a, sizeof(double) * nvec * h->ma, hipMemcpyHostToDevice, h->stream);
-    const int grid = std::min(nvec, h->slots);
-    if (e == hipSuccess)
+    if (e == hipSuccess && use_v2(h)) {
+        const int grid = std::min(nvec, h->slots2);
+        rc = dispatch2(h->var, [&]<int NT, int R>() {
+            hipLaunchKernelGGL((psk::eval_kernel2<NT, R, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream, h->dP2,
+                               h->S2, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
+            return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
+        });
+    } else if (e == hipSuccess) {
+        const int grid = std::min(nvec, h->slots);
         rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
             return launch_eval<NT, R, LDSB, MIX>(h, grid, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
         });
+    }
     if (e == hipSuccess && rc == PS_OK && alpha)
         e = hipMemcpyAsync(alpha, d_al, sizeof(double) * nvec * mf2, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess && rc == PS_OK && beta)

@@ -69,6 +69,18 @@ struct DevProblem {
     const int *fit_idx;
     const double *cg_first;
     const double *pole_beta, *pole_lambda;
+    // Pass schedule of the J-free kernels (ps_fit2.h).  There the point arrays above are in schedule
+    // order: positions grouped into passes of at most kPassTiles 16-point tiles, every relative
+    // lightcurve inside one pass, the three convexity rows at positions conv_q..conv_q+2 of the last
+    // pass, padding positions all zero.  Pass k covers tiles pass_tile0[k] .. pass_tile0[k+1]-1;
+    // its relative lightcurves are units pass_unit0[k] .. pass_unit0[k+1]-1, unit u covering
+    // pass-relative rows unit_rows[u] & 255 .. (unit_rows[u] >> 8) - 1 of lightcurve unit_lc[u].
+    int npass, nsched, conv_q, c00col;
+    const int *pass_tile0, *pass_unit0, *unit_rows, *unit_lc;
+    const int *qrel;   // [nsched] 1: the position holds a point of a relative lightcurve
+    const int *qperm;  // [nsched] the point's index in input order (-1: padding)
+    const int *fpos;   // [ma] the row column of a parameter (-1: fixed); column mfit = the residual
+    const double *qsl; // [nsched] sig*Lpoints of a relative lightcurve's point (mrqcof.c:94), else 0
 };
 
 struct Slots {
@@ -191,7 +203,8 @@ __device__ inline void blmatrix(double bet, double lam, double bl[3][3], double 
 }
 
 // Pole rotation into LDS once per evaluation (blmatrix.c:9-46).
-__device__ inline void blmatrix_lds(double bet, double lam, Shared &sh) {
+template <class SH>
+__device__ inline void blmatrix_lds(double bet, double lam, SH &sh) {
     if (threadIdx.x == 0) {
         double bl[3][3], dbl[2][3][3];
         blmatrix(bet, lam, bl, dbl);
@@ -212,7 +225,8 @@ __device__ inline void rotation_phase(double omg, double t, double phi0, double 
 }
 
 // Earth and Sun directions of point p in the body frame (bright.c:41-57 with matrix.c).
-__device__ inline void point_dirs(const DevProblem &P, const double *a, int p, const Shared &sh, double e[3],
+template <class SH>
+__device__ inline void point_dirs(const DevProblem &P, const double *a, int p, const SH &sh, double e[3],
                                   double e0[3], double rot[3]) {
     const double ee[3] = {gload(P.ex, p), gload(P.ey, p), gload(P.ez, p)};
     const double ee0[3] = {gload(P.sx, p), gload(P.sy, p), gload(P.sz, p)};
@@ -263,7 +277,8 @@ __device__ inline double phase_scale(const double *a, int nc, double alpha, doub
 // matrix.c:11-58): js[j] = S.de_j + U.de0_j, S and U the facet sums of dbr/de and dbr/de0.  With the
 // body-frame direction e = Rz(phi) B ee: de_j = Rz H_j Rz^T e for beta and lambda (H_j = dB_j B^T) and
 // de_omega = t (e_y, -e_x, 0) (dRz/dphi Rz^T is that generator); rot = (sin phi, cos phi, t).
-__device__ inline void spin_columns(const Shared &sh, const double e[3], const double e0[3], const double rot[3],
+template <class SH>
+__device__ inline void spin_columns(const SH &sh, const double e[3], const double e0[3], const double rot[3],
                                     const double S[3], const double U[3], double js[3]) {
     const double sf = rot[0], cf = rot[1], t = rot[2];
     const double ep[3] = {cf * e[0] - sf * e[1], sf * e[0] + cf * e[1], e[2]};  // Rz^T e
