@@ -14,10 +14,11 @@ per-period chi2 difference.  The roofline figures use the fit kernel's own durat
 HIP events recorded on the stream it is launched on.
 
     python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed] [--periods P]
-                    [--workload config1|config3]
+                    [--workload config1|config3|config4] [--batch-periods B]
 
 The default workload (config1) is the headline; config3 (a sparse survey fitted on 512 facets, Dsph
-read through L2) is a second shape, reported under its own metric name.
+read through L2) is a second shape, reported under its own metric name; config4 is the batched uneven
+workload (512 asteroids, one graph-captured persistent launch over a device work queue, FP64).
 """
 import argparse
 import json
@@ -32,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "trial-period LM fits/sec at Lmax=6, 288 facets: mixed vs FP64, same best period"
 METRIC_CONFIG3 = "trial-period LM fits/sec at Lmax=6, 512 facets, sparse survey (5000 points)"
+METRIC_CONFIG4 = "LM fits/sec of a batch of 512 asteroids (ndata 100-20000, Lmax 3/4/6, 288/512 facets), one persistent launch"
 PEAK_FP64 = 78.6   # MI355X dense FP64 matrix peak, TFLOP/s
 PEAK_FP32 = 157.3  # MI355X dense FP32 matrix peak, TFLOP/s
 ORACLE_CLI = os.path.join(ROOT, "oracle", "ps_oracle_cli")
@@ -151,6 +153,62 @@ def agreement(r64, rmx, windows, ndata):
             "fit_chi2_rel_err_median": float(f"{np.median(fit_errs):.3g}")}
 
 
+def bench_batch(args):
+    """The batched uneven workload: 512 seeded asteroids (synth.config4), a window of
+    --batch-periods trial periods x 10 poles around each true period, all (asteroid, period, pole)
+    fits as one ps_batch_run (a graph launch of persistent workgroups draining a device work queue).
+    A step replays the same batch; the flops count each fit's own iteration number."""
+    import numpy as np
+
+    from periodsearch_amd import Batch, PeriodSearch, synth
+
+    searches, conw, prob, freq, pole, f_eval = [], [], [], [], [], []
+    for i in range(synth.CONFIG4_N):
+        text, truth = synth.generate(synth.config4(i))
+        ps = PeriodSearch.from_text(text).prepare(device=0)
+        info = ps.info()
+        n_true = int(round((info["freq_start"] - 1 / truth["period_h"]) / info["freq_step"])) + 1
+        n = np.clip(n_true + np.arange(args.batch_periods) - args.batch_periods // 2, 1, info["nmax"])
+        searches.append(ps)
+        conw.append(info["conw_r0"])
+        freq.extend(ps.freq(np.repeat(n, 10)))
+        pole.extend(np.tile(np.arange(1, 11), len(n)))
+        prob.extend([i] * (10 * len(n)))
+        fs, fn = flops_per_eval(info)
+        f_eval.append(fs + fn)
+    conw, prob, freq, pole, f_eval = map(np.asarray, (conw, prob, freq, pole, f_eval))
+    with Batch(searches) as b:
+        for _ in range(args.warmup):
+            b.run(conw, prob, freq, pole)
+        b.kernel_time(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = b.run(conw, prob, freq, pole)
+        wall = time.perf_counter() - t0
+        kernel_ms, launches = b.kernel_time()
+    for ps in searches:
+        ps.close()
+    nfits = len(freq)
+    flops = float(np.sum((res["niter"] + 1) * f_eval[prob]))
+    avg_launch = kernel_ms * 1e-3 / launches
+    achieved = flops / avg_launch / 1e12
+    return {
+        "metric": METRIC_CONFIG4, "value": round(nfits * args.steps / wall, 2), "unit": "fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, periodsearch_amd/synth.py config4)",
+        "config": {"workload": f"config4_{synth.CONFIG4_N}asteroids", "asteroids": synth.CONFIG4_N,
+                   "fits_per_step": int(nfits), "periods_per_asteroid": args.batch_periods,
+                   "lm_iterations_min_mean_max": [int(res["niter"].min()), round(float(res["niter"].mean()), 2),
+                                                  int(res["niter"].max())],
+                   "launch": "one graph: queue reset + persistent workgroups over a device work queue"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64, 4), "traffic": None, "flops_per_step": flops,
+                     "avg_launch_ms": round(avg_launch * 1e3, 3)},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,9 +217,12 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="both (default): FP64 as the value, mixed on the same fits beside it")
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
-    ap.add_argument("--workload", choices=["config1", "config3"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config3", "config4"], default="config1",
                     help="config1: dense photometry, Lmax 6 on 288 facets, 2400 points (the headline); "
-                         "config3: sparse survey, Lmax 6 on 512 facets, 5000 points")
+                         "config3: sparse survey, Lmax 6 on 512 facets, 5000 points; "
+                         "config4: 512 asteroids of uneven size and shape as one batch (FP64)")
+    ap.add_argument("--batch-periods", type=int, default=2,
+                    help="config4: trial periods per asteroid per step (x 10 poles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline process")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -178,6 +239,10 @@ def main():
     import numpy as np
 
     from periodsearch_amd import PS_FP64, PS_MIXED, PeriodSearch, synth
+
+    if args.workload == "config4":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
 
     text, truth = synth.generate(getattr(synth, args.workload)())
     tmpdir = tempfile.mkdtemp(prefix="psbench_")
