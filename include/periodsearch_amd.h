@@ -148,6 +148,17 @@ int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, d
 int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, int njobs, const int *prob, const double *freq,
                  const int *pole, ps_fit_result *out);
 
+/* ps_fit_batch for callers that cannot allocate device memory (code under graph capture, tight serving
+   loops, harnesses that forbid hipMalloc); it has no counterpart in the reference.  freq, pole and out
+   are device arrays of nfits entries; the call allocates nothing and copies nothing between host and
+   device.  The per-workgroup buffers come from a static device array of 40 MiB per GPU (held until the
+   process ends), shared by every call on that GPU: calls that overlap on one GPU race.  Needs the
+   FP64 J-free kernels (PS_ERR_UNSUPPORTED otherwise); a shape whose buffers do not fit gives
+   PS_ERR_LIMIT.  A job with a pole outside 1..10 or a non-positive frequency yields niter = -1 and NaN
+   fields.  Returns after the launch has finished. */
+int ps_fit_batch_noalloc(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r,
+                         ps_fit_result *out);
+
 /* A batch of many problems (asteroids) on one device, for repeated runs.  The reference runs one
    asteroid per process and has no counterpart; the per-job body is period_search_BOINC.cpp:817-921.
    ps_batch_create: the problem table on the device and one pool of per-workgroup buffers sized for the

@@ -82,6 +82,7 @@ SIGNATURES = {
     "ps_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "ps_get_info": (C.c_int, [C.c_void_p, C.POINTER(PsInfo)]),
     "ps_fit_batch": (C.c_int, [C.c_void_p, C.c_int, DP, IP, C.c_double, C.POINTER(PsFitResult)]),
+    "ps_fit_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "ps_fit_multi": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, DP, C.c_int, IP, DP, IP, C.POINTER(PsFitResult)]),
     "ps_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "ps_batch_run": (C.c_int, [C.c_void_p, DP, C.c_int, IP, DP, IP, C.POINTER(PsFitResult)]),

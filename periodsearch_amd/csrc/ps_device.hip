@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/periodsearch_amd.h"
@@ -137,6 +138,29 @@ bool build_schedule(const std::vector<int> &lc_start, const std::vector<int> &lc
 }
 }  // namespace
 
+// The per-workgroup buffers of ps_fit_batch_noalloc: a file-scope device array, one copy per GPU (held
+// until the process ends), 40 MiB = 512 workgroup slots of the largest supported shape (mfit 56, 512
+// facets: 9968 doubles per slot).  Host code never uses the array's name as an address: each device's
+// copy is found once, at ps_prepare, with hipGetSymbolAddress.
+constexpr size_t kNoAllocDoubles = size_t(5) << 20;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices];  // each device's copy of scratch (nullptr: not looked up yet)
+std::mutex g_scratch_mu;
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return ps_fail(PS_ERR_DEVICE, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return PS_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return ps_fail(PS_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[device] = static_cast<double *>(p);
+    return PS_OK;
+}
+}  // namespace
+
 struct ps_handle {
     int device = 0;
     int precision = PS_FP64;
@@ -251,12 +275,6 @@ int set_conw(ps_handle *h, double conw_r) {
     for (int j = 0; j < 3; ++j) h->wq_host[h->ndata + j] = 1 / (sig * sig);
     HIPCHECK(hipMemcpyAsync(h->d_wq + h->ndata, h->wq_host.data() + h->ndata, 3 * sizeof(double),
                             hipMemcpyHostToDevice, h->stream));
-    if (h->v2) {
-        const int q = h->P2.conv_q;
-        for (int j = 0; j < 3; ++j) h->wq2_host[q + j] = 1 / (sig * sig);
-        HIPCHECK(hipMemcpyAsync(h->d_wq2 + q, h->wq2_host.data() + q, 3 * sizeof(double), hipMemcpyHostToDevice,
-                                h->stream));
-    }
     h->conw_loaded = conw_r;
     return PS_OK;
 }
@@ -315,6 +333,10 @@ extern "C" int ps_prepare(const ps_input *in, const ps_geometry *geo, int device
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ps_fail(PS_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return ps_fail(PS_ERR_ARG, "device %d out of range", device);
     HIPCHECK(hipSetDevice(device));
+    {
+        const int rc = noalloc_setup(device);  // this device's copy of the no-allocation scratch
+        if (rc != PS_OK) return rc;
+    }
 
     auto *h = new ps_handle();
     h->device = device;
@@ -773,6 +795,50 @@ extern "C" int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, 
     return PS_OK;
 }
 
+extern "C" int ps_fit_batch_noalloc(ps_handle *h, int nfits, const double *d_freq, const int *d_pole, double conw_r,
+                                    ps_fit_result *d_out) {
+    if (!h || nfits < 0 || (nfits && (!d_freq || !d_pole || !d_out)))
+        return ps_fail(PS_ERR_ARG, "ps_fit_batch_noalloc: bad argument");
+    if (nfits == 0) return PS_OK;
+    if (!(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_fit_batch_noalloc: conw_r must be positive");
+    if (!use_v2(h))
+        return ps_fail(PS_ERR_UNSUPPORTED, "ps_fit_batch_noalloc: needs the J-free kernels (precision PS_FP64, every "
+                                           "relative lightcurve <= %d points)", psk::kPassRows);
+    double *base = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch[h->device] : nullptr;
+    if (!base) return ps_fail(PS_ERR_DEVICE, "ps_fit_batch_noalloc: no scratch address for device %d", h->device);
+    // the slots carved from the array, each region rounded to 256 bytes
+    auto r32 = [](size_t n) { return (n + 31) & ~size_t(31); };
+    Slots2 T{};
+    T.a_stride = size_t(h->mfit) * h->mfit;
+    T.b_stride = size_t(h->mfit);
+    T.f_stride = size_t(h->nfac);
+    const size_t per_slot = 2 * T.a_stride + 2 * T.b_stride + psk::kStashPerSlot + T.f_stride;
+    size_t slots = std::min<size_t>(size_t(h->slots2), kNoAllocDoubles / (per_slot + 128));
+    if (slots < 1)
+        return ps_fail(PS_ERR_LIMIT, "ps_fit_batch_noalloc: one workgroup needs %zu bytes, the scratch holds %zu",
+                       per_slot * sizeof(double), kNoAllocDoubles * sizeof(double));
+    T.alpha = base;
+    T.beta = T.alpha + r32(T.a_stride * 2 * slots);
+    T.stash = T.beta + r32(T.b_stride * 2 * slots);
+    T.area = T.stash + r32(psk::kStashPerSlot * slots);
+    if (size_t(T.area + T.f_stride * slots - base) > kNoAllocDoubles)
+        return ps_fail(PS_ERR_LIMIT, "ps_fit_batch_noalloc: %zu slots do not fit the scratch", slots);
+    HIPCHECK(hipSetDevice(h->device));
+    const int grid = int(std::min<size_t>(size_t(nfits), slots));
+    const int rc = dispatch2(h->var, [&]<int NT, int R>() {
+        if (h->ls)
+            hipLaunchKernelGGL((psk::fit_kernel2<NT, R, true, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream,
+                               h->dP2, T, d_freq, d_pole, nfits, conw_r, d_out);
+        else
+            hipLaunchKernelGGL((psk::fit_kernel2<NT, R, false, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream,
+                               h->dP2, T, d_freq, d_pole, nfits, conw_r, d_out);
+        return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
+    });
+    if (rc != PS_OK) return ps_fail(rc, "ps_fit_batch_noalloc: launch failed: %s", hipGetErrorString(hipGetLastError()));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return PS_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Batches of many problems: one persistent launch draining a device work queue, captured in a graph.
 // ---------------------------------------------------------------------------------------------
@@ -894,13 +960,7 @@ extern "C" int ps_batch_run(ps_batch *b, const double *conw_r, int njobs, const 
     if (rc != PS_OK) return rc;
     if (njobs == 0) return PS_OK;
     HIPCHECK(hipSetDevice(b->device));
-    // the convexity rows' weight of every problem (each problem's own stream), then the job queue:
-    // problems without the Lambert column first, then those with it; each group longest first
-    for (int i = 0; i < nh; ++i) {
-        rc = set_conw(b->hs[i], conw_r[i]);
-        if (rc != PS_OK) return rc;
-    }
-    for (int i = 0; i < nh; ++i) HIPCHECK(hipStreamSynchronize(b->hs[i]->stream));
+    // the job queue: problems without the Lambert column first, then those with it; each group longest first
     std::vector<int> order(njobs);
     for (int j = 0; j < njobs; ++j) order[j] = j;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -1008,7 +1068,7 @@ extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, d
         const int grid = std::min(nvec, h->slots2);
         rc = dispatch2(h->var, [&]<int NT, int R>() {
             hipLaunchKernelGGL((psk::eval_kernel2<NT, R, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream, h->dP2,
-                               h->S2, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
+                               h->S2, d_a, nvec, conw_r, d_al, d_be, d_ch, d_y, d_dy);
             return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
         });
     } else if (e == hipSuccess) {

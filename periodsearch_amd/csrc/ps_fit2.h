@@ -342,11 +342,18 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
 struct PassRowData {
     double y_a, y_b, sl_a, sl_b, w_a, w_b;
 };
-__device__ inline PassRowData load_pass_rows(const DevProblem &P, int tile0) {
+// wqc: the weight of the three convexity rows, 1/sig^2 with sig = 1/conw_r (period_search_BOINC.cpp:721-726),
+// formed by the kernel from the call's conw_r.
+__device__ inline PassRowData load_pass_rows(const DevProblem &P, int tile0, double wqc) {
     const int c = threadIdx.x & 63;
-    const size_t q = size_t(tile0) * 16 + c;  // the point arrays are padded by a whole pass
-    return {gload(P.y, q), gload(P.y, q + 64), gload(P.qsl, q), gload(P.qsl, q + 64), gload(P.wq, q),
-            gload(P.wq, q + 64)};
+    const int q = tile0 * 16 + c;  // the point arrays are padded by a whole pass
+    const bool ca = q >= P.conv_q && q < P.conv_q + 3, cb = q + 64 >= P.conv_q && q + 64 < P.conv_q + 3;
+    return {gload(P.y, q), gload(P.y, q + 64), gload(P.qsl, q), gload(P.qsl, q + 64), ca ? wqc : gload(P.wq, q),
+            cb ? wqc : gload(P.wq, q + 64)};
+}
+__device__ inline double conv_weight(double conw_r) {
+    const double sig = 1 / conw_r;
+    return 1 / (sig * sig);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -549,8 +556,9 @@ __device__ __forceinline__ void stash_io(int nb, Prec<false>::V4 acc[kOwn], doub
 // returned.  raw_y / raw_dyda: the raw rows (ps_eval).
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool LS, int NFM>
-__device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh, double *area, double *stash,
-                                          double *alpha, double *beta, double *raw_y, double *raw_dyda) {
+__device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh, double wqc, double *area,
+                                          double *stash, double *alpha, double *beta, double *raw_y,
+                                          double *raw_dyda) {
     using Pd = Prec<false>;
     curv2(P, sh.a, sh, area);
     blmatrix_lds(sh.a[P.nc], sh.a[P.nc + 1], sh);
@@ -566,7 +574,7 @@ __device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh,
         __builtin_amdgcn_s_setprio(0);
         model_rows2<NT, R, LS>(P, sh, tile0, ntl, last, area, raw_y, raw_dyda);
         __builtin_amdgcn_s_setprio(2);
-        const PassRowData rd = load_pass_rows(P, tile0);
+        const PassRowData rd = load_pass_rows(P, tile0, wqc);
         __syncthreads();
         normalise_pass(P, sh, rd, P.pass_unit0[k], P.pass_unit0[k + 1]);
         __syncthreads();
@@ -702,6 +710,16 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
     const int tid = threadIdx.x;
     const int nc = P.nc, ma = P.ma, mfit = P.mfit;
     double *const area = sp.area;
+    const double wqc = conv_weight(conw_r);
+    if (pole < 1 || pole > PS_N_POLES || !(freq > 0) || !(conw_r > 0)) {
+        // a job the host did not check (ps_fit_batch_noalloc reads device arrays): no fit, niter = -1
+        if (tid == 0) {
+            const double nan = __builtin_nan("");
+            *out = {nan, nan, nan, nan, nan, nan, nan, -1, 0};
+        }
+        __syncthreads();
+        return;
+    }
     // initial parameters (:819-844)
     if (tid < ma) {
         double v;
@@ -759,7 +777,7 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             __syncthreads();
             // the starting point fills the current buffers, a trial point the other ones
             const bool other = !init;
-            const double chi = mrqcof2<NT, R, LS>(P, sh, area, sp.stash, (cur ^ other) ? sp.alpha1 : sp.alpha0,
+            const double chi = mrqcof2<NT, R, LS>(P, sh, wqc, area, sp.stash, (cur ^ other) ? sp.alpha1 : sp.alpha0,
                                                   (cur ^ other) ? sp.beta1 : sp.beta0, nullptr, nullptr);
             if (init) {
                 ochisq = chi;
@@ -908,8 +926,8 @@ __global__ void queue_reset(int *counter, int n) {
 // mrqcof for externally supplied parameter vectors (ps_eval), J-free.
 template <int NT, int R, int NFM>
 __global__ __launch_bounds__(kWG2, 2) void eval_kernel2(const DevProblem *__restrict__ Pp, Slots2 S, const double *avec,
-                                                     int nvec, double *alpha, double *beta, double *chisq,
-                                                     double *raw_y, double *raw_dyda) {
+                                                     int nvec, double conw_r, double *alpha, double *beta,
+                                                     double *chisq, double *raw_y, double *raw_dyda) {
     const DevProblem &P = *Pp;
     __shared__ Shared2<NFM> sh;
     const int tid = threadIdx.x;
@@ -920,7 +938,7 @@ __global__ __launch_bounds__(kWG2, 2) void eval_kernel2(const DevProblem *__rest
     for (int v = blockIdx.x; v < nvec; v += gridDim.x) {
         if (tid < P.ma) sh.a[tid] = avec[size_t(v) * P.ma + tid];
         __syncthreads();
-        const double c = mrqcof2<NT, R, true>(P, sh, area, stash, al, be,
+        const double c = mrqcof2<NT, R, true>(P, sh, conv_weight(conw_r), area, stash, al, be,
                                                   raw_y ? raw_y + size_t(v) * P.npts : nullptr,
                                                   raw_dyda ? raw_dyda + size_t(v) * P.npts * P.ma : nullptr);
         for (int it = tid; it < mfit * mfit; it += kWG2)
