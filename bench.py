@@ -51,8 +51,10 @@ def flops_per_eval(info):
     return n * 2 * nf * nc, n * (mf * (mf + 1) + 2 * mf)
 
 
-def traffic_per_launch(workload, precision):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
+def traffic_per_launch(workload, precision, kernel):
+    """HBM bytes per launch from the two counter passes committed under profiles/ (null when the kernel
+    form was not measured on this workload)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}_{kernel}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -238,7 +240,7 @@ def main():
 
     import numpy as np
 
-    from periodsearch_amd import PS_FP64, PS_MIXED, PeriodSearch, synth
+    from periodsearch_amd import PS_FP64, PS_KERNEL_JFREE, PS_MIXED, PeriodSearch, synth
 
     if args.workload == "config4":
         print(json.dumps(bench_batch(args)), flush=True)
@@ -257,6 +259,13 @@ def main():
     n0 = max(1, n_true - args.periods // 2)
     windows = [1 + (n0 - 1 + k * args.periods + np.arange(args.periods)) % info["nmax"] for k in range(args.steps)]
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    # the autotuner picks the FP64 kernel form on a sample of this problem's fits (outside the timed region)
+    tune = None
+    if "fp64" in precisions:
+        ps.set_precision(PS_FP64)
+        kern, ms_legacy, ms_jfree = ps.autotune(conw_r)
+        tune = {"kernel": "jfree" if kern == PS_KERNEL_JFREE else "legacy", "ms_per_fit_legacy": round(ms_legacy, 5),
+                "ms_per_fit_jfree": round(ms_jfree, 5) if ms_jfree else None}
     runs = {p: run_precision(ps, PS_FP64 if p == "fp64" else PS_MIXED, windows, args.warmup, conw_r)
             for p in precisions}
     ps.close()
@@ -276,8 +285,9 @@ def main():
             peak = PEAK_FP64
         else:  # shape contraction on FP32 MFMA, normal equations on FP64 MFMA: the time-weighted peak
             peak = (f_shape + f_ne) / (f_shape / PEAK_FP32 + f_ne / PEAK_FP64)
+        form = tune["kernel"] if (p == "fp64" and tune) else "legacy"
         return {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": traffic_per_launch(workload, p),
+                "frac": round(achieved / peak, 4), "traffic": traffic_per_launch(workload, p, form), "kernel": form,
                 "flops_per_fit": f_fit, "avg_launch_ms": round(x["avg_launch_s"] * 1e3, 3)}
 
     line = {
@@ -298,6 +308,7 @@ def main():
                    "fits_per_step": per_step, "precision": head, "first_period_index": int(windows[0][0]),
                    "trial_periods_per_s": round(r["value"] / 10, 2)},
         "roofline": roofline(head),
+        "autotune": tune,
     }
     if len(precisions) == 2:
         m = runs["mixed"]

@@ -44,6 +44,11 @@ extern "C" {
 
 #define PS_N_POLES 10 /* period_search/constants.h:11 */
 
+/* Kernel form of single-problem FP64 fits (ps_set_kernel, ps_autotune). */
+#define PS_KERNEL_AUTO 0   /* the J-free kernels where they accept the problem, else the legacy ones */
+#define PS_KERNEL_LEGACY 1 /* 512-lane workgroups, one per CU, the Jacobian rows through a per-fit buffer */
+#define PS_KERNEL_JFREE 2  /* 256-lane workgroups, two per CU, lightcurve-aligned passes of 128 rows in LDS */
+
 /* Precision of the fit kernels. */
 #define PS_FP64 0  /* FP64 throughout (the reference arithmetic) */
 #define PS_MIXED 1 /* FP32 facet sums / normal matrix, FP64 phase, chi2 test and solve */
@@ -135,6 +140,18 @@ void ps_geometry_free(ps_geometry *geo);
    supported; the reference allows up to MAX_N_PAR = 200 parameters and MAX_N_FAC = 1000 facets). */
 int ps_prepare(const ps_input *in, const ps_geometry *geo, int device, ps_handle **out);
 int ps_set_precision(ps_handle *h, int precision);
+/* Sets the kernel form of single-problem FP64 fits (PS_KERNEL_*); batches and ps_fit_batch_noalloc always
+   run the J-free form.  No counterpart in the reference (one CPU build). */
+int ps_set_kernel(ps_handle *h, int kernel);
+/* The kernel form the next single-problem FP64 fit will use (PS_KERNEL_LEGACY or PS_KERNEL_JFREE). */
+int ps_get_kernel(const ps_handle *h);
+/* The autotuner: times both kernel forms on a sample of the problem's own fits (the first trial periods
+   x 10 poles at convexity weight conw_r, 2 x the J-free form's resident workgroups, the second of two
+   launches of each form), sets the faster one and reports it in *kernel and the two times per fit
+   (legacy, J-free) in ms_per_fit[2] (either may be NULL).  Results are kept per problem shape (ncoef,
+   nfac, npts, mfit, Lambert column, iteration cap) for the life of the process.  Mixed precision and
+   problems the J-free form does not accept get PS_KERNEL_LEGACY without timing. */
+int ps_autotune(ps_handle *h, double conw_r, int *kernel, double *ms_per_fit);
 int ps_get_info(const ps_handle *h, ps_info *info);
 /* Runs nfits independent fits (freq[i] in 1/h, pole[i] in 1..10) with convexity weight conw_r. */
 int ps_fit_batch(ps_handle *h, int nfits, const double *freq, const int *pole, double conw_r, ps_fit_result *out);

@@ -13,6 +13,9 @@ EXE_PATH = os.path.join(_HERE, "bin", "period_search_amd")
 PS_OK = 0
 PS_FP64 = 0
 PS_MIXED = 1
+PS_KERNEL_AUTO = 0
+PS_KERNEL_LEGACY = 1
+PS_KERNEL_JFREE = 2
 N_POLES = 10
 
 ERRORS = {
@@ -80,6 +83,9 @@ SIGNATURES = {
     "ps_geometry_free": (None, [C.POINTER(PsGeometry)]),
     "ps_prepare": (C.c_int, [C.POINTER(PsInput), C.POINTER(PsGeometry), C.c_int, C.POINTER(C.c_void_p)]),
     "ps_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "ps_set_kernel": (C.c_int, [C.c_void_p, C.c_int]),
+    "ps_get_kernel": (C.c_int, [C.c_void_p]),
+    "ps_autotune": (C.c_int, [C.c_void_p, C.c_double, IP, DP]),
     "ps_get_info": (C.c_int, [C.c_void_p, C.POINTER(PsInfo)]),
     "ps_fit_batch": (C.c_int, [C.c_void_p, C.c_int, DP, IP, C.c_double, C.POINTER(PsFitResult)]),
     "ps_fit_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),

@@ -99,6 +99,21 @@ class PeriodSearch:
     def set_precision(self, precision):
         L.check(self._lib.ps_set_precision(self._need(), precision), "ps_set_precision")
 
+    def set_kernel(self, kernel):
+        """Kernel form of single-problem FP64 fits: PS_KERNEL_AUTO, _LEGACY or _JFREE."""
+        L.check(self._lib.ps_set_kernel(self._need(), kernel), "ps_set_kernel")
+
+    def kernel(self):
+        return L.check(self._lib.ps_get_kernel(self._need()), "ps_get_kernel")
+
+    def autotune(self, conw_r):
+        """Times both kernel forms on a sample of this problem's fits and keeps the faster: returns
+        (kernel form, ms per fit of the legacy form, ms per fit of the J-free form)."""
+        k = C.c_int()
+        ms = (C.c_double * 2)()
+        L.check(self._lib.ps_autotune(self._need(), float(conw_r), C.byref(k), ms), "ps_autotune")
+        return k.value, ms[0], ms[1]
+
     def info(self):
         inf = L.PsInfo()
         L.check(self._lib.ps_get_info(self._need(), C.byref(inf)), "ps_get_info")

@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -182,6 +185,7 @@ struct ps_handle {
     Slots2 S2{};
     int slots2 = 0;
     int nseg = 0;
+    int kernel = PS_KERNEL_AUTO;  // kernel form of single-problem FP64 fits (ps_set_kernel / ps_autotune)
     std::vector<double> wq2_host;
     double *d_wq2 = nullptr;
     std::vector<void *> owned;
@@ -266,8 +270,11 @@ int launch_fit2(ps_handle *h, int grid, int nfits, double conw_r) {
     return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
 }
 
-// The J-free kernels run the FP64 fits and evaluations of a problem whose schedule they accept.
-bool use_v2(const ps_handle *h) { return h->v2 && h->precision == PS_FP64; }
+// The J-free kernels can run the FP64 fits and evaluations of a problem whose schedule they accept
+// (batches and ps_fit_batch_noalloc use nothing else); single-problem fits use them unless the kernel
+// form was set (or tuned) to the materialised-Jacobian kernels.
+bool v2_ok(const ps_handle *h) { return h->v2 && h->precision == PS_FP64; }
+bool use_v2(const ps_handle *h) { return v2_ok(h) && h->kernel != PS_KERNEL_LEGACY; }
 
 int set_conw(ps_handle *h, double conw_r) {
     if (h->conw_loaded == conw_r) return PS_OK;
@@ -756,7 +763,7 @@ extern "C" int ps_fit_multi(ps_handle *const *hs, int nh, const double *conw_r, 
     HIPCHECK(hipSetDevice(hs[0]->device));
     // FP64 problems of the J-free kernels: one persistent launch over a device work queue
     bool all_v2 = true;
-    for (int i = 0; i < nh; ++i) all_v2 = all_v2 && use_v2(hs[i]);
+    for (int i = 0; i < nh; ++i) all_v2 = all_v2 && v2_ok(hs[i]);
     if (all_v2) {
         ps_batch *b = nullptr;
         rc = ps_batch_create(hs, nh, &b);
@@ -801,7 +808,7 @@ extern "C" int ps_fit_batch_noalloc(ps_handle *h, int nfits, const double *d_fre
         return ps_fail(PS_ERR_ARG, "ps_fit_batch_noalloc: bad argument");
     if (nfits == 0) return PS_OK;
     if (!(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_fit_batch_noalloc: conw_r must be positive");
-    if (!use_v2(h))
+    if (!v2_ok(h))
         return ps_fail(PS_ERR_UNSUPPORTED, "ps_fit_batch_noalloc: needs the J-free kernels (precision PS_FP64, every "
                                            "relative lightcurve <= %d points)", psk::kPassRows);
     double *base = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch[h->device] : nullptr;
@@ -837,6 +844,92 @@ extern "C" int ps_fit_batch_noalloc(ps_handle *h, int nfits, const double *d_fre
     if (rc != PS_OK) return ps_fail(rc, "ps_fit_batch_noalloc: launch failed: %s", hipGetErrorString(hipGetLastError()));
     HIPCHECK(hipStreamSynchronize(h->stream));
     return PS_OK;
+}
+
+extern "C" int ps_set_kernel(ps_handle *h, int kernel) {
+    if (!h) return ps_fail(PS_ERR_ARG, "ps_set_kernel: null handle");
+    if (kernel != PS_KERNEL_AUTO && kernel != PS_KERNEL_LEGACY && kernel != PS_KERNEL_JFREE)
+        return ps_fail(PS_ERR_ARG, "ps_set_kernel: unknown kernel form %d", kernel);
+    if (kernel == PS_KERNEL_JFREE && !h->v2)
+        return ps_fail(PS_ERR_UNSUPPORTED, "ps_set_kernel: the J-free kernels do not accept this problem (a relative "
+                                           "lightcurve longer than %d points)", psk::kPassRows);
+    h->kernel = kernel;
+    return PS_OK;
+}
+
+namespace {
+// Kernel forms chosen by ps_autotune, per problem shape, for the life of the process.
+struct TuneKey {
+    int ncoef, nfac, npts, mfit, ls, iters;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(ncoef, nfac, npts, mfit, ls, iters) < std::tie(o.ncoef, o.nfac, o.npts, o.mfit, o.ls, o.iters);
+    }
+};
+std::map<TuneKey, std::array<double, 2>> g_tuned;  // ms per fit of the two forms
+std::mutex g_tuned_mu;
+}  // namespace
+
+extern "C" int ps_autotune(ps_handle *h, double conw_r, int *kernel, double *ms_per_fit) {
+    if (!h || !(conw_r > 0)) return ps_fail(PS_ERR_ARG, "ps_autotune: bad argument");
+    std::array<double, 2> ms{0, 0};
+    int choice = PS_KERNEL_LEGACY;
+    if (h->precision == PS_FP64 && h->v2) {
+        const TuneKey key{h->ncoef, h->nfac, h->npts, h->mfit, int(h->ls), h->P.n_iter_max};
+        bool known = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tuned_mu);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) {
+                ms = it->second;
+                known = true;
+            }
+        }
+        if (!known) {
+            // a sample of the problem's own fits: the first trial periods x 10 poles, two launches of each
+            // form at full occupancy of the J-free form (the second one timed)
+            const int nfits = 2 * h->slots2;
+            std::vector<double> freq(nfits);
+            std::vector<int> pole(nfits);
+            for (int i = 0; i < nfits; ++i) {
+                freq[i] = h->freq_start - (i / PS_N_POLES) * h->freq_step;
+                pole[i] = 1 + i % PS_N_POLES;
+            }
+            std::vector<ps_fit_result> out(nfits);
+            const int saved = h->kernel;
+            const double km = h->kernel_ms;
+            const int kl = h->launches;
+            for (int f = 0; f < 2; ++f) {
+                h->kernel = f == 0 ? PS_KERNEL_LEGACY : PS_KERNEL_JFREE;
+                for (int rep = 0; rep < 2; ++rep) {
+                    const double before = h->kernel_ms;
+                    const int rc = ps_fit_batch(h, nfits, freq.data(), pole.data(), conw_r, out.data());
+                    if (rc != PS_OK) {
+                        h->kernel = saved;
+                        return rc;
+                    }
+                    ms[f] = (h->kernel_ms - before) / nfits;
+                }
+            }
+            h->kernel = saved;
+            h->kernel_ms = km;
+            h->launches = kl;
+            std::lock_guard<std::mutex> lk(g_tuned_mu);
+            g_tuned[key] = ms;
+        }
+        choice = ms[1] <= ms[0] ? PS_KERNEL_JFREE : PS_KERNEL_LEGACY;
+    }
+    h->kernel = choice;
+    if (kernel) *kernel = choice;
+    if (ms_per_fit) {
+        ms_per_fit[0] = ms[0];
+        ms_per_fit[1] = ms[1];
+    }
+    return PS_OK;
+}
+
+extern "C" int ps_get_kernel(const ps_handle *h) {
+    if (!h) return ps_fail(PS_ERR_ARG, "ps_get_kernel: null handle");
+    return use_v2(h) ? PS_KERNEL_JFREE : PS_KERNEL_LEGACY;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -884,7 +977,7 @@ extern "C" int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out) {
     for (int i = 0; i < nh; ++i) {
         if (!hs[i]) return ps_fail(PS_ERR_ARG, "ps_batch_create: problem %d is null", i);
         if (hs[i]->device != hs[0]->device) return ps_fail(PS_ERR_ARG, "ps_batch_create: problems on different devices");
-        if (!use_v2(hs[i]))
+        if (!v2_ok(hs[i]))
             return ps_fail(PS_ERR_UNSUPPORTED, "ps_batch_create: problem %d needs the FP64 J-free kernels (precision FP64, "
                                                "every relative lightcurve <= %d points)", i, psk::kPassRows);
     }
