@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "trial-period LM fits/sec at Lmax=6, 288 facets: mixed vs FP64, same best period"
 METRIC_CONFIG3 = "trial-period LM fits/sec at Lmax=6, 512 facets, sparse survey (5000 points)"
+METRIC_CONFIG2 = "model evaluations/sec (forward dyda + J^T W J, J^T W r without storing J) at Lmax=6, 288 facets"
 METRIC_CONFIG4 = "LM fits/sec of a batch of 512 asteroids (ndata 100-20000, Lmax 3/4/6, 288/512 facets), one persistent launch"
 PEAK_FP64 = 78.6   # MI355X dense FP64 matrix peak, TFLOP/s
 PEAK_FP32 = 157.3  # MI355X dense FP32 matrix peak, TFLOP/s
@@ -155,6 +156,85 @@ def agreement(r64, rmx, windows, ndata):
             "fit_chi2_rel_err_median": float(f"{np.median(fit_errs):.3g}")}
 
 
+def bench_diff(args):
+    """The differentiation workload at the production shape: the 4096 parameter vectors of
+    synth.config2_vectors, one ps_eval each -- the forward model with dyda of every parameter and the
+    normal equations accumulated without storing J (the J-free kernels; the materialised-Jacobian
+    kernels are timed beside them).  A step evaluates all 4096 vectors.  Accuracy against the reference
+    core's values committed in tests/golden/diff_config2.npz (chi2, beta, alpha.u of all vectors; bright()
+    rows of 16 vectors at 64 points) and against central finite differences of the GPU's own model."""
+    import numpy as np
+
+    from periodsearch_amd import PS_KERNEL_JFREE, PS_KERNEL_LEGACY, PeriodSearch, synth
+
+    text, _ = synth.generate(synth.config1())
+    ps = PeriodSearch.from_text(text).prepare(device=0)
+    info = ps.info()
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "diff_config2.npz"))
+    conw = float(fx["conw_r"])
+    vecs = synth.config2_vectors(ps)
+    rates = {}
+    for name, form in (("legacy", PS_KERNEL_LEGACY), ("jfree", PS_KERNEL_JFREE)):
+        ps.set_kernel(form)
+        for _ in range(args.warmup):
+            ps.eval(vecs, conw)
+        ps.kernel_time(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            got = ps.eval(vecs, conw)
+        wall = time.perf_counter() - t0
+        kms, launches = ps.kernel_time()
+        rates[name] = {"wall": wall, "avg_launch_s": kms * 1e-3 / launches, "got": got}
+    r = rates["jfree"]
+    got = r["got"]
+    rel = lambda e, ref: float(np.max(e / np.abs(ref)))  # noqa: E731
+    acc = {"chisq_vs_reference_max_rel": rel(np.abs(got["chisq"] - fx["chisq"]), fx["chisq"]),
+           "beta_vs_reference_max_rel": float(np.max(np.abs(got["beta"] - fx["beta"]).max(1) /
+                                                     np.abs(fx["beta"]).max(1))),
+           "alpha_u_vs_reference_max_rel": float(np.max(np.abs(got["alpha"] @ fx["u"] - fx["alpha_u"]).max(1) /
+                                                        np.abs(fx["alpha_u"]).max(1)))}
+    raw = ps.eval(vecs[fx["bright_vec"]], conw, raw=True)
+    d = raw["dyda"][:, fx["bright_pts"], :]
+    acc["dyda_vs_reference_bright_max_rel"] = float(np.max(np.abs(d - fx["dyda"]).max((0, 1)) /
+                                                           np.abs(fx["dyda"]).max((0, 1))))
+    # central differences (relative step 1e-6; shape coefficients: of their basis scale; omega 1e-9 abs)
+    ma, nc = info["ma"], info["ncoef"]
+    span = np.abs(ps.geometry()["dsph"]).max(0)
+    pts = np.array([0, 321, 1999, 2000, 2399])
+    fd_err = 0.0
+    for v in (5, 3000):
+        a = vecs[v]
+        an = ps.eval(a[None], conw, raw=True)["dyda"][0][pts]
+        h = 1e-6 * np.maximum(np.abs(a), 1.0)
+        h[:nc] = 1e-6 * np.maximum(np.abs(a[:nc]), 1.0 / span)
+        h[nc + 2] = 1e-9
+        plus, minus = np.tile(a, (ma, 1)), np.tile(a, (ma, 1))
+        plus[np.arange(ma), np.arange(ma)] += h
+        minus[np.arange(ma), np.arange(ma)] -= h
+        fd = ((ps.eval(plus, conw, raw=True)["ymod"][:, pts] - ps.eval(minus, conw, raw=True)["ymod"][:, pts]) /
+              (2 * h[:, None])).T
+        fd_err = max(fd_err, float(np.max(np.abs(fd - an).max(0) / np.maximum(np.abs(an).max(0), 1e-300))))
+    acc["dyda_vs_central_differences_max_rel"] = fd_err
+    ps.close()
+    f_eval = sum(flops_per_eval(info))
+    nvec = len(vecs)
+    achieved = nvec * f_eval / r["avg_launch_s"] / 1e12
+    return {
+        "metric": METRIC_CONFIG2, "value": round(nvec * args.steps / r["wall"], 2), "unit": "evaluations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, periodsearch_amd/synth.py config1 + config2_vectors)",
+        "config": {"workload": "config2_lmax6_288fac_ndata2400_4096vectors", "vectors_per_step": nvec,
+                   "ndata": info["ndata"], "mfit": info["mfit"], "kernel": "jfree",
+                   "legacy_evaluations_per_s": round(nvec * args.steps / rates["legacy"]["wall"], 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64, 4), "traffic": None, "flops_per_eval": f_eval,
+                     "avg_launch_ms": round(r["avg_launch_s"] * 1e3, 3)},
+        "accuracy": {k: float(f"{v:.3g}") for k, v in acc.items()},
+        "cpu_baseline": None,
+    }
+
+
 def bench_batch(args):
     """The batched uneven workload: 512 seeded asteroids (synth.config4), a window of
     --batch-periods trial periods x 10 poles around each true period, all (asteroid, period, pole)
@@ -219,8 +299,9 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="both (default): FP64 as the value, mixed on the same fits beside it")
     ap.add_argument("--periods", type=int, default=256, help="trial periods per step (x 10 poles)")
-    ap.add_argument("--workload", choices=["config1", "config3", "config4"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config2", "config3", "config4"], default="config1",
                     help="config1: dense photometry, Lmax 6 on 288 facets, 2400 points (the headline); "
+                         "config2: 4096 model evaluations with dyda and the normal equations at the config1 shape; "
                          "config3: sparse survey, Lmax 6 on 512 facets, 5000 points; "
                          "config4: 512 asteroids of uneven size and shape as one batch (FP64)")
     ap.add_argument("--batch-periods", type=int, default=2,
@@ -244,6 +325,9 @@ def main():
 
     if args.workload == "config4":
         print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.workload == "config2":
+        print(json.dumps(bench_diff(args)), flush=True)
         return
 
     text, truth = synth.generate(getattr(synth, args.workload)())

@@ -202,11 +202,12 @@ int ps_scan(ps_handle *h, int n_from, int n_to, double conw_r, ps_line *lines);
 /* mrqcof for nvec parameter vectors a[nvec*ma]: alpha[nvec*mfit*mfit], beta[nvec*mfit], chisq[nvec]
    (normalised normal equations, fitted parameters only); when ymod/dyda are non-NULL also the raw
    bright()/conv() outputs per point, ymod[nvec*npts], dyda[nvec*npts*ma], before lightcurve
-   normalisation.  Any output pointer may be NULL. */
+   normalisation.  Any output pointer may be NULL.  The device staging is kept by the handle between
+   calls (grown when a call needs more).  The kernel time counts in ps_kernel_time. */
 int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, double *alpha, double *beta, double *chisq,
             double *ymod, double *dyda);
-/* Device time of the fit kernels since the last reset, measured with HIP events on the library's
-   stream: total milliseconds and number of launches. */
+/* Device time of the fit and evaluation kernels since the last reset, measured with HIP events on the
+   library's stream: total milliseconds and number of launches. */
 int ps_kernel_time(ps_handle *h, double *ms, int *launches, int reset);
 void ps_release(ps_handle *h);
 

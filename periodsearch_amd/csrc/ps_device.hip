@@ -186,6 +186,10 @@ struct ps_handle {
     int slots2 = 0;
     int nseg = 0;
     int kernel = PS_KERNEL_AUTO;  // kernel form of single-problem FP64 fits (ps_set_kernel / ps_autotune)
+    struct EvalBuffers {          // ps_eval's device staging, kept between calls
+        double *a = nullptr, *al = nullptr, *be = nullptr, *ch = nullptr, *y = nullptr, *dy = nullptr;
+        size_t ca = 0, cal = 0, cbe = 0, cch = 0, cy = 0, cdy = 0;
+    } ev;
     std::vector<double> wq2_host;
     double *d_wq2 = nullptr;
     std::vector<void *> owned;
@@ -1126,6 +1130,26 @@ extern "C" int ps_batch_kernel_time(ps_batch *b, double *ms, int *launches, int 
     return PS_OK;
 }
 
+namespace {
+// A device buffer of the handle that grows on demand (ps_eval's staging) and is freed by ps_release.
+int grow(ps_handle *h, double *&p, size_t &cap, size_t n) {
+    if (n <= cap) return PS_OK;
+    if (p) {
+        (void)hipFree(p);
+        h->owned.erase(std::remove(h->owned.begin(), h->owned.end(), static_cast<void *>(p)), h->owned.end());
+        p = nullptr;
+        cap = 0;
+    }
+    if (hipMalloc(&p, sizeof(double) * n) != hipSuccess) {
+        p = nullptr;
+        return ps_fail(PS_ERR_NOMEM, "ps_eval: device allocation of %zu doubles failed", n);
+    }
+    h->owned.push_back(p);
+    cap = n;
+    return PS_OK;
+}
+}  // namespace
+
 extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, double *alpha, double *beta,
                        double *chisq, double *ymod, double *dyda) {
     if (!h || nvec < 0 || (nvec && !a)) return ps_fail(PS_ERR_ARG, "ps_eval: bad argument");
@@ -1136,54 +1160,45 @@ extern "C" int ps_eval(ps_handle *h, int nvec, const double *a, double conw_r, d
     int rc = set_conw(h, conw_r);
     if (rc == PS_OK) rc = ensure_scratch(h, use_v2(h));
     if (rc != PS_OK) return rc;
-    const size_t mf2 = size_t(h->mfit) * h->mfit;
-    double *d_a = nullptr, *d_al = nullptr, *d_be = nullptr, *d_ch = nullptr, *d_y = nullptr, *d_dy = nullptr;
-    auto cleanup = [&]() {
-        hipFree(d_a);
-        hipFree(d_al);
-        hipFree(d_be);
-        hipFree(d_ch);
-        hipFree(d_y);
-        hipFree(d_dy);
-    };
-    bool ok = hipMalloc(&d_a, sizeof(double) * nvec * h->ma) == hipSuccess &&
-              (!alpha || hipMalloc(&d_al, sizeof(double) * nvec * mf2) == hipSuccess) &&
-              (!beta || hipMalloc(&d_be, sizeof(double) * nvec * h->mfit) == hipSuccess) &&
-              (!chisq || hipMalloc(&d_ch, sizeof(double) * nvec) == hipSuccess) &&
-              (!ymod || hipMalloc(&d_y, sizeof(double) * nvec * h->npts) == hipSuccess) &&
-              (!dyda || hipMalloc(&d_dy, sizeof(double) * nvec * h->npts * h->ma) == hipSuccess);
-    if (!ok) {
-        cleanup();
-        return ps_fail(PS_ERR_NOMEM, "ps_eval: device allocation failed");
-    }
-    hipError_t e = hipMemcpyAsync(d_a, a, sizeof(double) * nvec * h->ma, hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess && use_v2(h)) {
+    const size_t mf2 = size_t(h->mfit) * h->mfit, nv = size_t(nvec);
+    // staging buffers kept by the handle: allocated on first use, grown when a call needs more
+    auto &E = h->ev;
+    if ((rc = grow(h, E.a, E.ca, nv * h->ma)) != PS_OK || (alpha && (rc = grow(h, E.al, E.cal, nv * mf2)) != PS_OK) ||
+        (beta && (rc = grow(h, E.be, E.cbe, nv * h->mfit)) != PS_OK) || (chisq && (rc = grow(h, E.ch, E.cch, nv)) != PS_OK) ||
+        (ymod && (rc = grow(h, E.y, E.cy, nv * h->npts)) != PS_OK) ||
+        (dyda && (rc = grow(h, E.dy, E.cdy, nv * h->npts * h->ma)) != PS_OK))
+        return rc;
+    double *d_al = alpha ? E.al : nullptr, *d_be = beta ? E.be : nullptr, *d_ch = chisq ? E.ch : nullptr;
+    double *d_y = ymod ? E.y : nullptr, *d_dy = dyda ? E.dy : nullptr;
+    HIPCHECK(hipMemcpyAsync(E.a, a, sizeof(double) * nv * h->ma, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(hipEventRecord(h->ev0, h->stream));
+    if (use_v2(h)) {
         const int grid = std::min(nvec, h->slots2);
         rc = dispatch2(h->var, [&]<int NT, int R>() {
             hipLaunchKernelGGL((psk::eval_kernel2<NT, R, kMaxFac>), dim3(grid), dim3(psk::kWG2), 0, h->stream, h->dP2,
-                               h->S2, d_a, nvec, conw_r, d_al, d_be, d_ch, d_y, d_dy);
+                               h->S2, E.a, nvec, conw_r, d_al, d_be, d_ch, d_y, d_dy);
             return hipGetLastError() == hipSuccess ? PS_OK : PS_ERR_DEVICE;
         });
-    } else if (e == hipSuccess) {
+    } else {
         const int grid = std::min(nvec, h->slots);
         rc = dispatch(h->var, h->precision == PS_MIXED, [&]<int NT, int R, bool LDSB, bool MIX>() {
-            return launch_eval<NT, R, LDSB, MIX>(h, grid, d_a, nvec, d_al, d_be, d_ch, d_y, d_dy);
+            return launch_eval<NT, R, LDSB, MIX>(h, grid, E.a, nvec, d_al, d_be, d_ch, d_y, d_dy);
         });
     }
-    if (e == hipSuccess && rc == PS_OK && alpha)
-        e = hipMemcpyAsync(alpha, d_al, sizeof(double) * nvec * mf2, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess && rc == PS_OK && beta)
-        e = hipMemcpyAsync(beta, d_be, sizeof(double) * nvec * h->mfit, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess && rc == PS_OK && chisq)
-        e = hipMemcpyAsync(chisq, d_ch, sizeof(double) * nvec, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess && rc == PS_OK && ymod)
-        e = hipMemcpyAsync(ymod, d_y, sizeof(double) * nvec * h->npts, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess && rc == PS_OK && dyda)
-        e = hipMemcpyAsync(dyda, d_dy, sizeof(double) * nvec * h->npts * h->ma, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    cleanup();
-    if (e != hipSuccess) return ps_fail(PS_ERR_DEVICE, "ps_eval: %s", hipGetErrorString(e));
-    return rc;
+    if (rc != PS_OK) return ps_fail(rc, "ps_eval: launch failed: %s", hipGetErrorString(hipGetLastError()));
+    HIPCHECK(hipEventRecord(h->ev1, h->stream));
+    if (alpha) HIPCHECK(hipMemcpyAsync(alpha, d_al, sizeof(double) * nv * mf2, hipMemcpyDeviceToHost, h->stream));
+    if (beta) HIPCHECK(hipMemcpyAsync(beta, d_be, sizeof(double) * nv * h->mfit, hipMemcpyDeviceToHost, h->stream));
+    if (chisq) HIPCHECK(hipMemcpyAsync(chisq, d_ch, sizeof(double) * nv, hipMemcpyDeviceToHost, h->stream));
+    if (ymod) HIPCHECK(hipMemcpyAsync(ymod, d_y, sizeof(double) * nv * h->npts, hipMemcpyDeviceToHost, h->stream));
+    if (dyda)
+        HIPCHECK(hipMemcpyAsync(dyda, d_dy, sizeof(double) * nv * h->npts * h->ma, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->kernel_ms += ms;
+    h->launches += 1;
+    return PS_OK;
 }
 
 extern "C" int ps_kernel_time(ps_handle *h, double *ms, int *launches, int reset) {

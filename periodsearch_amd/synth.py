@@ -234,3 +234,26 @@ def config4(i, **kw):
                 fit_nrows=nrows, stop_condition=0.001, n_iter_min=10)
     base.update(kw)
     return SynthSpec(**base)
+
+
+def config2_vectors(ps, nvec=4096, seed=1002):
+    """Parameter vectors of the differentiation workload at a prepared problem's shape (0-based, ma
+    entries): shape coefficients N(0, 0.3) in units of each basis function's largest facet value (the
+    unnormalised Legendre functions reach ~1e4 at l = m = 6, so raw N(0, 0.3) coefficients overflow
+    exp()), c00 from the initial ellipsoid, poles uniform on the sphere (beta as colatitude), periods
+    U(2, 20) h, the phase-function and Lambert parameters at the input's initial values, cls = 1."""
+    rng = np.random.default_rng(seed)
+    geo = ps.geometry()
+    inp = ps.input
+    nc = len(geo["cg_first"])
+    span = np.abs(geo["dsph"]).max(0)
+    a = np.zeros((nvec, nc + 8))
+    a[:, 0] = geo["cg_first"][0]
+    a[:, 1:nc] = rng.normal(0, 0.3, (nvec, nc - 1)) / span[1:]
+    a[:, nc] = np.arccos(rng.uniform(-1, 1, nvec))
+    a[:, nc + 1] = rng.uniform(0, 2 * np.pi, nvec)
+    a[:, nc + 2] = 24 * 2 * np.pi / rng.uniform(2, 20, nvec)  # omega [rad/d]
+    a[:, nc + 3:nc + 6] = [inp.par[0], inp.par[1], inp.par[2]]
+    a[:, nc + 6] = np.log(inp.cl)
+    a[:, nc + 7] = 1.0
+    return a
