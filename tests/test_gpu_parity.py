@@ -6,7 +6,7 @@ equations within 1e-11 of the matrix scale; fitted-period index and best-pole se
 import numpy as np
 import pytest
 
-from periodsearch_amd import PeriodSearch, synth
+from periodsearch_amd import PS_KERNEL_JFREE, PS_KERNEL_LEGACY, PeriodSearch, synth
 from tests import oracle_binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -73,51 +73,33 @@ def test_fit_batch_matches_oracle(small):
 
 # ---- the Lambert weight fitted too (the kernels with the Lambert and Lommel-Seeliger columns), and the
 # coefficient layouts the other shapes leave out: Lmax 5 (two MFMA tiles + four VALU columns) and Lmax 2
-# (9 coefficients zero-padded to four tiles, Dsph read from L2).  With the Lambert weight free the fits
-# get ill-conditioned: a rejected last trial on a near-singular system sets the reported area, and a
-# runaway weight (log cl near 470) overflows the reference's ymod*dave_l, after which its Gauss-Jordan
-# fails on every step; at log cl near 85 the normalised Lambert column is cancellation noise, which any
-# other summation order turns into another trajectory.  A quantity therefore matches the oracle to 1e-9
-# or, where the oracle itself moves by more under a second valid FP64 rounding (the FMA-contracted
-# build) or a 1e-12 relative change of the convexity weight, lies within twice that movement.  A fit
-# whose rms the oracle cannot reproduce to 1e-6 that way (a runaway weight) is degenerate: it is not
-# compared, and at most half of the fits of a case may be.
-@pytest.mark.parametrize("lmax,cl_free", [(3, True), (5, False), (5, True), (2, True)])
-def test_fit_free_lambert_and_layouts_match_oracle(lmax, cl_free):
-    text, _ = synth.generate(synth.config0(seed=21, fit_lmax=lmax, cl_free=cl_free, n_dense=4, dense_points=30,
+# (9 coefficients zero-padded to four tiles).  With the Lambert weight free, fits can run away (a
+# Lambert weight of e^85 and more turns the normalised Lambert column into cancellation noise that any
+# other summation order follows differently), so the inputs are seeded ones whose fits the oracle
+# reproduces to 1e-11 under its own perturbations (the FMA-contracted build, a 1e-12 relative change
+# of the convexity weight; tests/golden has no such inputs).  Every fit is compared, to 1e-9, for both
+# kernel forms.
+@pytest.mark.parametrize("kernel", [PS_KERNEL_JFREE, PS_KERNEL_LEGACY], ids=["jfree", "legacy"])
+@pytest.mark.parametrize("lmax,cl_free,seed", [(3, True, 35), (5, False, 23), (5, True, 25), (2, True, 22)])
+def test_fit_free_lambert_and_layouts_match_oracle(lmax, cl_free, seed, kernel):
+    text, _ = synth.generate(synth.config0(seed=seed, fit_lmax=lmax, cl_free=cl_free, n_dense=4, dense_points=30,
                                            stop_condition=15))
     orc = ob.OracleCtx.from_text(text)
-    orc_fma = ob.OracleCtx.from_text(text, fma=True)
-    orc_eps = ob.OracleCtx.from_text(text)
     assert orc.ia()[-2] == int(cl_free)  # the cl flag reached the fit
     ps = PeriodSearch.from_text(text).prepare(device=0)
+    ps.set_kernel(kernel)
     try:
         conw_r = ps.info()["conw_r0"]
         orc.set_conw(conw_r)
-        orc_fma.set_conw(conw_r)
         n = np.array([1, 1, 2, 3])
         poles = np.array([1, 6, 3, 9])
         got = ps.fit_batch(ps.freq(n), poles, conw_r)
-        degenerate = 0
         for i in range(len(n)):
-            f, m = orc.freq(int(n[i])), int(poles[i])
-            ref = orc.fit(f, m)
-            alts = [orc_fma.fit(f, m)]
-            for eps in (1e-12, -1e-12):
-                orc_eps.set_conw(conw_r * (1 + eps))
-                alts.append(orc_eps.fit(f, m))
+            ref = orc.fit(orc.freq(int(n[i])), int(poles[i]))
             assert got["niter"][i] == ref["niter"]
-            if max(_rel(x["dev_new"], ref["dev_new"]) for x in alts) > 1e-6:
-                degenerate += 1
-                continue
             for key, rkey in (("dev", "dev_new"), ("chisq", "chisq"), ("prd", "prd"), ("totarea", "totarea")):
-                d = _rel(got[key][i], ref[rkey])
-                spread = max(_rel(x[rkey], ref[rkey]) for x in alts)
-                assert d < 1e-9 or d <= 2 * spread, (i, key, d, spread)
-            for key in ("la", "be"):
-                spread = max(abs(x[key] - ref[key]) for x in alts)
-                assert abs(got[key][i] - ref[key]) < max(1e-6, 2 * spread)
-        assert degenerate <= len(n) // 2
+                assert _rel(got[key][i], ref[rkey]) < 1e-9, (i, key, got[key][i], ref[rkey])
+            assert abs(got["la"][i] - ref["la"]) < 1e-6 and abs(got["be"][i] - ref["be"]) < 1e-6
     finally:
         ps.close()
 
