@@ -1,6 +1,8 @@
-"""The on-disk formats end to end on the reference's own real inputs (a sample of the 100 sparse-survey
-period_search_in files the reference ships, tests/golden/real_inputs/): parsing and the setup tables
-against the oracle, bit for bit.  The GPU scans of the same files are in test_gpu_parity.py."""
+"""The on-disk formats end to end on the reference's own real inputs (all 100 sparse-survey
+period_search_in files the reference ships in period_search/input_period_search/, copied to
+tests/golden/real_inputs/): parsing and the setup tables against the oracle, bit for bit.  GPU scans
+of three of them are in test_gpu_parity.py, the executable in test_executable.py and
+test_gpu_executable.py."""
 import glob
 import os
 
@@ -14,8 +16,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REAL = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "real_inputs", "period_search_in_*")))
 
 
-def test_real_input_sample_present():
-    assert len(REAL) == 12
+def test_real_inputs_present():
+    assert len(REAL) == 100
 
 
 @pytest.mark.parametrize("path", REAL, ids=[os.path.basename(p) for p in REAL])
