@@ -304,6 +304,8 @@ def main():
                          "config2: 4096 model evaluations with dyda and the normal equations at the config1 shape; "
                          "config3: sparse survey, Lmax 6 on 512 facets, 5000 points; "
                          "config4: 512 asteroids of uneven size and shape as one batch (FP64)")
+    ap.add_argument("--kernel", choices=["auto", "legacy", "jfree"], default="auto",
+                    help="FP64 kernel form: auto (the autotuner picks, outside the timed region), or a fixed one")
     ap.add_argument("--batch-periods", type=int, default=2,
                     help="config4: trial periods per asteroid per step (x 10 poles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -321,7 +323,7 @@ def main():
 
     import numpy as np
 
-    from periodsearch_amd import PS_FP64, PS_KERNEL_JFREE, PS_MIXED, PeriodSearch, synth
+    from periodsearch_amd import PS_FP64, PS_KERNEL_JFREE, PS_KERNEL_LEGACY, PS_MIXED, PeriodSearch, synth
 
     if args.workload == "config4":
         print(json.dumps(bench_batch(args)), flush=True)
@@ -347,9 +349,13 @@ def main():
     tune = None
     if "fp64" in precisions:
         ps.set_precision(PS_FP64)
-        kern, ms_legacy, ms_jfree = ps.autotune(conw_r)
-        tune = {"kernel": "jfree" if kern == PS_KERNEL_JFREE else "legacy", "ms_per_fit_legacy": round(ms_legacy, 5),
-                "ms_per_fit_jfree": round(ms_jfree, 5) if ms_jfree else None}
+        if args.kernel == "auto":
+            kern, ms_legacy, ms_jfree = ps.autotune(conw_r)
+            tune = {"kernel": "jfree" if kern == PS_KERNEL_JFREE else "legacy",
+                    "ms_per_fit_legacy": round(ms_legacy, 5), "ms_per_fit_jfree": round(ms_jfree, 5) if ms_jfree else None}
+        else:
+            ps.set_kernel(PS_KERNEL_JFREE if args.kernel == "jfree" else PS_KERNEL_LEGACY)
+            tune = {"kernel": args.kernel, "fixed": True}
     runs = {p: run_precision(ps, PS_FP64 if p == "fp64" else PS_MIXED, windows, args.warmup, conw_r)
             for p in precisions}
     ps.close()
