@@ -39,7 +39,10 @@ struct Shared2 {
     double cg[64];    // accepted parameters
     double atry[64];
     double nx[NFM], ny[NFM], nz[NFM], acl[NFM], acls[NFM];  // normals, Area*cl, Area*cls
-    double da[64];
+    union {
+        double da[64];       // the LM step: the Gauss-Jordan right-hand side, then its solution
+        double csum[2][64];  // phase C: the column sums of two consecutive lightcurves
+    };
     double bl[9], dbl[18], hbl[18];  // pole rotation, its derivatives, the pole-frame generators
     double scal[4];
     int job;  // the queue kernel: the job claimed by the workgroup
@@ -360,36 +363,42 @@ __device__ inline double conv_weight(double conw_r) {
 // Phase C of one pass: every relative lightcurve (unit) of the pass is normalised in place
 // (mrqcof.c:53-101): dave_c = the sum of its raw column c, ave = the sum of its model values,
 // coef = sig*Lpoints/ave; dyda_c <- coef*(dyda_c - (ymod*dave_c)*(1/ave)), the c00 column 0, and the
-// residual column y - coef*ymod.  All four waves work on each unit: each forms the column sums
-// itself (lane c: column c, over the rows in order, so every wave holds the same bits) and the
-// per-row factors (lane i: pass rows i and 64 + i), then normalises rows r0 + w, r0 + w + 4, ...
+// residual column y - coef*ymod.  The column sums are split over the waves: lane (pl, kg) of wave w
+// adds column 16w + pl over the rows r0 + kg, r0 + kg + 4, ..., the four row classes are added across
+// the lane groups in a fixed order, ((0 + 1) + (2 + 3)), and the 64 sums go through LDS (sh.csum, two
+// units in flight, so one barrier per unit suffices).  Each wave then normalises rows r0 + w, r0 + w + 4, ... with lane c
+// on column c, taking the per-row factors from lane i (pass rows i and 64 + i).
 // ---------------------------------------------------------------------------------------------
 template <int NFM>
 __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, const PassRowData &rd, int u0, int u1) {
     const int c = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int pl = c & 15, kg = c >> 4;
     const int mfit = P.mfit, c00col = P.c00col;
     const int *unit_rows = P.unit_rows;
     const bool incol = c <= mfit, zero_col = c == c00col, res_col = c == mfit;
+    // the summed column, clamped: lanes past mfit add a valid column and their sums are not used
+    const int scol = min(16 * wave + pl, mfit);
     const double ym_a = sh.u.rows[c * kJP + mfit], ym_b = sh.u.rows[(64 + c) * kJP + mfit];
     for (int u = u0; u < u1; ++u) {
         const int rg = uload(unit_rows, u), r0 = rg & 255, r1 = rg >> 8;
-        // four interleaved partial sums (a shorter dependency chain), added in a fixed order
-        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        int r = r0;
-        for (; r + 3 < r1; r += 4) {
-            s0 += incol ? sh.u.rows[r * kJP + c] : 0.0;
-            s1 += incol ? sh.u.rows[(r + 1) * kJP + c] : 0.0;
-            s2 += incol ? sh.u.rows[(r + 2) * kJP + c] : 0.0;
-            s3 += incol ? sh.u.rows[(r + 3) * kJP + c] : 0.0;
+        // class kg: rows r0 + kg, r0 + kg + 4, ... of the whole groups of four, in order; the rows past
+        // the last whole group go to class 0, in order
+        const int nfull = (r1 - r0) >> 2;
+        double sk = 0;
+        for (int i = 0; i < nfull; ++i) sk += sh.u.rows[(r0 + 4 * i + kg) * kJP + scol];
+        for (int r = r0 + 4 * nfull; r < r1; ++r) {
+            const double x = sh.u.rows[r * kJP + scol];
+            sk += kg == 0 ? x : 0.0;
         }
-        for (; r < r1; ++r) s0 += incol ? sh.u.rows[r * kJP + c] : 0.0;
-        const double s = (s0 + s1) + (s2 + s3);
-        const double ave = read_lane(s, mfit);
+        double *cs = sh.csum[(u - u0) & 1];
+        const double sw = sum_groups(sk);  // (class 0 + class 1) + (class 2 + class 3)
+        if (kg == 0) cs[16 * wave + pl] = sw;
+        // every wave has read the rows of the unit and written its sums
+        __syncthreads();
+        const double s = cs[c], ave = cs[mfit];
         const double rave = 1 / ave;
         const double coef_a = rd.sl_a / ave, coef_b = rd.sl_b / ave;
         const double res_a = rd.y_a - coef_a * ym_a, res_b = rd.y_b - coef_b * ym_b;
-        // every wave has read the rows of the unit before any of them is rewritten
-        __syncthreads();
 #pragma unroll 4
         for (int r = r0 + wave; r < r1; r += kWaves2) {
             const bool lo = r < 64;
@@ -451,28 +460,24 @@ __device__ __forceinline__ void gram_pass_w(const DevProblem &P, const Shared2<N
     constexpr int used = own_used(NB, W);
     const int lane = threadIdx.x & 63, pl = lane & 15, kg = lane >> 4;
     const int ncol = P.mfit + 1;
-    bool has[4];  // this lane's column of each column block exists
+    // the lane's column of each block, clamped: a lane past the last column reads column 0, and its
+    // products land only in rows / columns of G that are not stored
+    int col[4];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) has[cb] = 16 * cb + pl < ncol;
-    // software pipeline: the operands of K-step s + 1 are loaded while step s runs on the MFMA
-    auto load = [&](int s, double v[4], double &w) {
-        const int row = 4 * s + kg;
-        w = __shfl(s < 16 ? rd.w_a : rd.w_b, row & 63);
+    for (int cb = 0; cb < 4; ++cb) col[cb] = (16 * cb + pl < ncol) ? 16 * cb + pl : 0;
+    const double *base = sh.u.rows + kg * kJP;
+    for (int t = 0; t < ntl; ++t) {  // one tile: four K-steps of four rows
+        const double wt = t < 4 ? rd.w_a : rd.w_b;
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-            v[cb] = ((used & (1 << cb)) && has[cb]) ? sh.u.rows[row * kJP + 16 * cb + pl] : 0.0;
-    };
-    const int ksteps = 4 * ntl;
-    double vn[4], wn;
-    load(0, vn, wn);
-#pragma unroll 4
-    for (int s = 0; s < ksteps; ++s) {
-        double v[4], w = wn;
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int r = 16 * t + 4 * s4;  // K-step rows r .. r + 3, the lane's row r + kg
+            const double w = __shfl(wt, (r + kg) & 63);
+            double v[4];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) v[cb] = vn[cb];
-        if (s + 1 < ksteps) load(s + 1, vn, wn);
+            for (int cb = 0; cb < 4; ++cb) v[cb] = (used & (1 << cb)) ? base[r * kJP + col[cb]] : 0.0;
 #pragma unroll
-        for (int b = 0; b < o.n; ++b) acc[b] = Pd::mma(v[o.bi[b]], w * v[o.bj[b]], acc[b]);
+            for (int b = 0; b < o.n; ++b) acc[b] = Pd::mma(v[o.bi[b]], w * v[o.bj[b]], acc[b]);
+        }
     }
 }
 
@@ -566,7 +571,9 @@ __device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh,
     const int npass = P.npass;
     const int nb = (P.mfit + 16) >> 4;  // 16-column blocks covering mfit + 1 columns
     for (int k = 0; k < npass; ++k) {
-        const int tile0 = P.pass_tile0[k], ntl = P.pass_tile0[k + 1] - tile0;
+        // the schedule through the scalar cache: wave-uniform trip counts and row ranges
+        const int tile0 = uload(P.pass_tile0, k), ntl = uload(P.pass_tile0, k + 1) - tile0;
+        const int u0 = uload(P.pass_unit0, k), u1 = uload(P.pass_unit0, k + 1);
         const bool last = k == npass - 1;
         // the facet loop issues at the lowest wave priority: when the other workgroup on the CU is in a
         // latency-bound phase (lightcurve sums, Gram, Gauss-Jordan), its instructions go first and the
@@ -576,7 +583,7 @@ __device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh,
         __builtin_amdgcn_s_setprio(2);
         const PassRowData rd = load_pass_rows(P, tile0, wqc);
         __syncthreads();
-        normalise_pass(P, sh, rd, P.pass_unit0[k], P.pass_unit0[k + 1]);
+        normalise_pass(P, sh, rd, u0, u1);
         __syncthreads();
         // the accumulators live only through phase D: parked in the slot between passes
         Pd::V4 acc[kOwn];
@@ -746,13 +753,37 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
     double iter_diff = 1e40, dev_old = 1e30, dev_new = 0;
     double chck0 = 0, chck1 = 0, chck2 = 0;
     int niter = 0, nfail = 0;
-    while (((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min)) {
+    // the LM controls and state are wave-uniform.  The compiler cannot see that for the state: its
+    // values meet at the joins of lane-predicated stores (tid < ma), so settle() re-asserts them after
+    // each such region; the loop exit and the accept / reject decisions then branch on scalars.
+    const int n_iter_max = __builtin_amdgcn_readfirstlane(P.n_iter_max);
+    const int n_iter_min = __builtin_amdgcn_readfirstlane(P.n_iter_min);
+    const double iter_diff_max = uniform_double(P.iter_diff_max), alamda_start = uniform_double(P.alamda_start);
+    const double alamda_incr = uniform_double(P.alamda_incr);
+    auto settle = [&] {
+        alamda = uniform_double(alamda);
+        ochisq = uniform_double(ochisq);
+        chisq = uniform_double(chisq);
+        rchisq = uniform_double(rchisq);
+        iter_diff = uniform_double(iter_diff);
+        dev_old = uniform_double(dev_old);
+        dev_new = uniform_double(dev_new);
+        chck0 = uniform_double(chck0);
+        chck1 = uniform_double(chck1);
+        chck2 = uniform_double(chck2);
+        niter = __builtin_amdgcn_readfirstlane(niter);
+        nfail = __builtin_amdgcn_readfirstlane(nfail);
+        cur = __builtin_amdgcn_readfirstlane(cur);
+    };
+    for (;;) {
+        settle();
+        if (!(((niter < n_iter_max) && (iter_diff > iter_diff_max)) || (niter < n_iter_min))) break;
         // One mrqcof per pass: the first pass evaluates the starting point (mrqmin.c:34-52), later
         // passes solve the damped system and evaluate the trial point (mrqmin.c:53-79).
         const bool init = alamda < 0;
         int err = 0;
         if (init) {
-            alamda = P.alamda_start;
+            alamda = alamda_start;
             if (tid < ma) sh.a[tid] = sh.cg[tid];
         } else {
             for (int it = tid; it < mfit * mfit; it += kWG2) {
@@ -762,7 +793,7 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             }
             for (int j = tid; j < mfit; j += kWG2) sh.da[j] = (cur ? sp.beta1 : sp.beta0)[j];
             __syncthreads();
-            err = gauss_jordan2(sh, mfit);
+            err = __builtin_amdgcn_readfirstlane(gauss_jordan2(sh, mfit));
             __syncthreads();
             if (err == 0) {
                 if (tid < mfit) {
@@ -777,8 +808,9 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             __syncthreads();
             // the starting point fills the current buffers, a trial point the other ones
             const bool other = !init;
-            const double chi = mrqcof2<NT, R, LS>(P, sh, wqc, area, sp.stash, (cur ^ other) ? sp.alpha1 : sp.alpha0,
-                                                  (cur ^ other) ? sp.beta1 : sp.beta0, nullptr, nullptr);
+            const double chi = uniform_double(mrqcof2<NT, R, LS>(P, sh, wqc, area, sp.stash,
+                                                                 (cur ^ other) ? sp.alpha1 : sp.alpha0,
+                                                                 (cur ^ other) ? sp.beta1 : sp.beta0, nullptr, nullptr));
             if (init) {
                 ochisq = chi;
                 chisq = chi;  // a failed first step then reports this fit's own starting chi2
@@ -786,11 +818,11 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             } else {
                 chisq = chi;
                 if (chisq < ochisq) {
-                    alamda = alamda / P.alamda_incr;
+                    alamda = alamda / alamda_incr;
                     cur ^= 1;
                     if (tid < ma) sh.cg[tid] = sh.atry[tid];
                 } else {
-                    alamda = P.alamda_incr * alamda;
+                    alamda = alamda_incr * alamda;
                     chisq = ochisq;
                 }
             }
@@ -798,6 +830,7 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
         } else {
             ++nfail;
         }
+        settle();
         if (init) continue;
         ++niter;
         if (niter == 1 || chisq < ochisq) {
@@ -817,13 +850,14 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
                 }
             }
             __syncthreads();
-            chck0 = sh.scal[1];
-            chck1 = sh.scal[2];
-            chck2 = sh.scal[3];
+            chck0 = uniform_double(sh.scal[1]);
+            chck1 = uniform_double(sh.scal[2]);
+            chck2 = uniform_double(sh.scal[3]);
             rchisq = chisq - (chck0 * chck0 + chck1 * chck1 + chck2 * chck2) * (conw_r * conw_r);
             __syncthreads();
         }
-        dev_new = sqrt(rchisq / P.nobs);
+        settle();
+        dev_new = sqrt(rchisq / __builtin_amdgcn_readfirstlane(P.nobs));
         if (dev_old - dev_new > 1e-10) {
             iter_diff = dev_old - dev_new;
             dev_old = dev_new;
@@ -865,15 +899,19 @@ __global__ __launch_bounds__(kWG2, 2) void fit_kernel2(const DevProblem *__restr
 // is empty.  A job names its problem; the problem's coefficient layout selects the fit body.  The
 // slot buffers are sized for the largest problem.
 // ---------------------------------------------------------------------------------------------
-// A pointer the whole wave holds, made provably uniform (its fields are then read with scalar loads).
-// readfirstlane works on 32-bit ints: both halves go through unsigned, so the low half is not
-// sign-extended into the high one.
+// A pointer to global memory the whole wave holds, made provably uniform.  readfirstlane works on
+// 32-bit ints: both halves go through unsigned, so the low half is not sign-extended into the high
+// one.  The result is cast from the global address space, so the compiler reads its fields with
+// global (or scalar) loads; through a generic pointer it would treat every field it loads as
+// varying across lanes (a generic address may name per-lane scratch), and all LM control flow
+// that depends on the problem's fields would branch per lane.
 template <class T>
 __device__ inline T *uniform_ptr(T *p) {
     const unsigned long long v = reinterpret_cast<unsigned long long>(p);
     const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(v));
     const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(v >> 32));
-    return reinterpret_cast<T *>((static_cast<unsigned long long>(hi) << 32) | lo);
+    auto *g = reinterpret_cast<__attribute__((address_space(1))) T *>((static_cast<unsigned long long>(hi) << 32) | lo);
+    return (T *)g;
 }
 
 struct QueueJob {

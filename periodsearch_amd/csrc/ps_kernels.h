@@ -915,6 +915,15 @@ __device__ inline PivotCand pivot_wave_max(PivotCand c) {
     return c;
 }
 
+// A double every lane holds (an LDS broadcast), made provably wave-uniform: the LM control flow that
+// depends on it then branches on scalar conditions and keeps its state in scalar registers.
+__device__ inline double uniform_double(double x) {
+    const unsigned long long b = __double_as_longlong(x);
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(b))));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(b >> 32))));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 __device__ inline double read_lane(double x, int l) {  // x of lane l (l wave-uniform)
     const unsigned long long b = __double_as_longlong(x);
     const unsigned lo = __builtin_amdgcn_readlane(unsigned(b), l), hi = __builtin_amdgcn_readlane(unsigned(b >> 32), l);
@@ -1103,7 +1112,27 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
         double iter_diff = 1e40, dev_old = 1e30, dev_new = 0;
         double chck0 = 0, chck1 = 0, chck2 = 0;
         int niter = 0, nfail = 0;
-        while (((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min)) {
+        // the LM state is wave-uniform; its values meet at the joins of lane-predicated stores
+        // (tid < ma), so settle() re-asserts it after each such region: the loop exit and the
+        // accept / reject decisions then branch on scalars
+        auto settle = [&] {
+            alamda = uniform_double(alamda);
+            ochisq = uniform_double(ochisq);
+            chisq = uniform_double(chisq);
+            rchisq = uniform_double(rchisq);
+            iter_diff = uniform_double(iter_diff);
+            dev_old = uniform_double(dev_old);
+            dev_new = uniform_double(dev_new);
+            chck0 = uniform_double(chck0);
+            chck1 = uniform_double(chck1);
+            chck2 = uniform_double(chck2);
+            niter = __builtin_amdgcn_readfirstlane(niter);
+            nfail = __builtin_amdgcn_readfirstlane(nfail);
+            cur = __builtin_amdgcn_readfirstlane(cur);
+        };
+        for (;;) {
+            settle();
+            if (!(((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min))) break;
             // One mrqcof per pass (a single inlined copy): the first pass of a fit evaluates the
             // starting point (mrqmin.c:34-52) and goes straight on to the step; later passes solve the
             // damped system and evaluate the trial point (mrqmin.c:53-79).
@@ -1125,7 +1154,7 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
                     if (MIX) sh.b0[j] = sh.da[j];
                 }
                 __syncthreads();
-                err = gauss_jordan(sh, mfit);
+                err = __builtin_amdgcn_readfirstlane(gauss_jordan(sh, mfit));
                 __syncthreads();
                 if (MIX && err == 0) refine_solution(sh, mfit);
                 if (err == 0) {
@@ -1141,8 +1170,8 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
                 __syncthreads();
                 // the starting point fills the current buffers, a trial point the other ones
                 const bool other = !init;
-                const double chi = mrqcof<NT, R, LDSB, MIX, LS>(P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0,
-                                                            (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr);
+                const double chi = uniform_double(mrqcof<NT, R, LDSB, MIX, LS>(
+                    P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0, (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr));
                 if (init) {
                     ochisq = chi;
                     chisq = chi;  // a failed first step then reports this fit's own starting chi2
@@ -1162,6 +1191,7 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
             } else {
                 ++nfail;
             }
+            settle();
             if (init) continue;  // the starting point is evaluated; the first step follows
             ++niter;
             if (niter == 1 || chisq < ochisq) {
@@ -1177,12 +1207,13 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
                     sh.scal[1 + tid] = s;
                 }
                 __syncthreads();
-                chck0 = sh.scal[1];
-                chck1 = sh.scal[2];
-                chck2 = sh.scal[3];
+                chck0 = uniform_double(sh.scal[1]);
+                chck1 = uniform_double(sh.scal[2]);
+                chck2 = uniform_double(sh.scal[3]);
                 rchisq = chisq - (chck0 * chck0 + chck1 * chck1 + chck2 * chck2) * (conw_r * conw_r);
                 __syncthreads();
             }
+            settle();
             dev_new = sqrt(rchisq / P.nobs);
             if (dev_old - dev_new > 1e-10) {
                 iter_diff = dev_old - dev_new;
