@@ -97,6 +97,15 @@ __device__ inline double wave_sum(double x) {  // sum over the 64 lanes, the sam
     return sum_groups(sum_row16(x));
 }
 
+// The sum over the four 16-lane groups, complete in the lanes of group 0 only (the others hold partial
+// sums), in the association of sum_groups, ((g0 + g1) + (g2 + g3)): the lanes are exchanged through the
+// LDS crossbar (ds_bpermute), which leaves the VALU issue slots to the other waves on the SIMD.
+__device__ inline double sum_groups_lo(double x) {
+    const int l = threadIdx.x & 63;
+    x = x + __shfl(x, (l + 16) & 63);
+    return x + __shfl(x, (l + 32) & 63);
+}
+
 // sum_i Area_i * v_i over the facets (the convexity check, period_search_BOINC.cpp:875-880, and the
 // total area), by wave 0: lane l adds facets l, l + 64, ..., then the lanes are added in a fixed tree.
 __device__ inline double facet_sum(const double *area, const double *v, int nf) {
@@ -124,23 +133,24 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
     const double p1 = a[nc + 3], p2 = a[nc + 4];
     const int nchunk = P.nf >> 2;
     const int tl = 2 * wave;  // the wave's first tile, pass-relative
-    int fcol[NT];             // the row column of this lane's shape column of each MFMA tile
+    // the row column of this lane's shape column of each MFMA tile; a coefficient that is not fitted
+    // (or past nc) goes to column mfit, which the same wave rewrites afterwards with the model value
+    // or residual of the point (its LDS writes are done in program order)
+    int fcol[NT];
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt) {
         const int k = 16 * tt + pl;
-        fcol[tt] = (k < nc) ? P.fpos[k] : -1;
+        const int c = (k < nc) ? P.fpos[k] : -1;
+        fcol[tt] = c >= 0 ? c : mfit;
     }
-    auto is_conv = [&](int q) { return q >= P.conv_q && q < P.conv_q + 3; };
-    if (tl < ntl) {
-        double e[kTP][3], e0[kTP][3], rot[kTP][3], alp[kTP], eph[kTP], dph[kTP];
-        bool valid[kTP];
-        TileSums<NT, R, false> ts[kTP];
+    // the wave's tiles: a pair, or the last tile of a pass alone (its facet loop then carries one
+    // tile, and the SIMD time of the missing one goes to the other workgroup on the CU)
+    auto tiles = [&]<int KT>() {
+        double e[KT][3], e0[KT][3], rot[KT][3], alp[KT], eph[KT], dph[KT];
+        TileSums<NT, R, false> ts[KT];
 #pragma unroll
-        for (int h = 0; h < kTP; ++h) {
-            // a missing second tile reads the next positions (the arrays are padded by a tile) and is
-            // not written
+        for (int h = 0; h < KT; ++h) {
             const int q = (tile0 + tl + h) * 16 + pl;
-            valid[h] = tl + h < ntl;
             point_dirs(P, a, q, sh, e[h], e0[h], rot[h]);
             alp[h] = gload(P.alpha, q);
             eph[h] = exp(-alp[h] / p2);
@@ -187,7 +197,7 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
 #pragma unroll
             for (int r = 0; r < RP; ++r) Brn[r] = dpr[r];
 #pragma unroll
-            for (int h = 0; h < kTP; ++h) {
+            for (int h = 0; h < KT; ++h) {
                 TileSums<NT, R, false> &s = ts[h];
                 const double mu = e[h][0] * n0 + e[h][1] * n1 + e[h][2] * n2;
                 const double mu0 = e0[h][0] * n0 + e0[h][1] * n1 + e0[h][2] * n2;
@@ -212,49 +222,48 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
                 for (int r = 0; r < R; ++r) s.racc[r] += X * Br[r];
             }
         }
-        double scale[kTP], dphp[kTP][3];
+        double scale[KT], dphp[KT][3];
 #pragma unroll
-        for (int h = 0; h < kTP; ++h) scale[h] = phase_scale(a, nc, alp[h], eph[h], dph[h], dphp[h]);
+        for (int h = 0; h < KT; ++h) scale[h] = phase_scale(a, nc, alp[h], eph[h], dph[h], dphp[h]);
         // shape columns: register r of lane l holds point row(r, kg), coefficient 16 tt + pl
 #pragma unroll
-        for (int h = 0; h < kTP; ++h) {
+        for (int h = 0; h < KT; ++h) {
             const int tile = tl + h;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = Pr::row(r, kg);
                 const double sc = __shfl(scale[h], i);
                 const int q = (tile0 + tile) * 16 + i;
-                const bool wr = valid[h] && !is_conv(q);
                 const int orig = raw_dyda ? gload(P.qperm, q) : -1;
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) {
                     const double v = sc * ts[h].acc[tt][r];
-                    if (wr && fcol[tt] >= 0) sh.u.rows[(16 * tile + i) * kJP + fcol[tt]] = v;
-                    if (wr && orig >= 0 && 16 * tt + pl < nc) raw_dyda[size_t(orig) * ma + 16 * tt + pl] = v;
+                    sh.u.rows[(16 * tile + i) * kJP + fcol[tt]] = v;
+                    if (orig >= 0 && 16 * tt + pl < nc) raw_dyda[size_t(orig) * ma + 16 * tt + pl] = v;
                 }
             }
         }
-        // the remaining columns of point pl (every lane group holds the same values)
+        // the remaining columns of point pl (complete in the lanes of group 0, which write them)
 #pragma unroll
-        for (int h = 0; h < kTP; ++h) {
+        for (int h = 0; h < KT; ++h) {
             TileSums<NT, R, false> &s = ts[h];
             const int tile = tl + h;
             const int q = (tile0 + tile) * 16 + pl;
-            const double br = sum_groups(s.br);
+            const double br = sum_groups_lo(s.br);
             double S[3], U[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                S[k] = sum_groups(s.S[k]);
-                U[k] = sum_groups(s.U[k]);
+                S[k] = sum_groups_lo(s.S[k]);
+                U[k] = sum_groups_lo(s.U[k]);
             }
             double d1 = 0, d2 = 0;
             if constexpr (LS) {
-                d1 = sum_groups(s.d1);
-                d2 = sum_groups(s.d2);
+                d1 = sum_groups_lo(s.d1);
+                d2 = sum_groups_lo(s.d2);
             }
             double racc[R > 0 ? R : 1];
 #pragma unroll
-            for (int r = 0; r < R; ++r) racc[r] = sum_groups(s.racc[r]);
+            for (int r = 0; r < R; ++r) racc[r] = sum_groups_lo(s.racc[r]);
             constexpr int kOther = R + 9;  // R leftover shape columns, beta/lambda/omega, 3 phase, cl, cls, ymod
             double ov[kOther];
             {
@@ -271,11 +280,11 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
                 ov[R + 8] = br * scale[h];
             }
             auto ocol = [&](int k) { return (k < R) ? 16 * NT + k : nc + (k - R); };
-            if (kg == 0 && valid[h] && !is_conv(q)) {
+            if (kg == 0) {
                 double *row = sh.u.rows + (16 * tile + pl) * kJP;
 #pragma unroll
                 for (int k = 0; k < kOther - 1; ++k) {
-                    const int c = P.fpos[ocol(k)];
+                    const int c = uload(P.fpos, ocol(k));
                     if (c >= 0) row[c] = ov[k];
                 }
                 const double ymod = ov[kOther - 1];
@@ -291,11 +300,17 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
                 }
             }
         }
-    }
+        };
+    if (tl + 1 < ntl)
+        tiles.template operator()<2>();
+    else if (tl < ntl)
+        tiles.template operator()<1>();
     if (last) {
         // convexity rows (conv.c:12-29): res_c = sum Area*n_c, dres_k = sum Area*Dsph_k*n_c; lane (k, q)
         // sums coefficient k (k == nc: the plain sums) over facets q, q+4, ..., the quarters are added
-        // across the lane quad.  Their residual is 0 - res_c (y = 0).
+        // across the lane quad.  Their residual is 0 - res_c (y = 0).  The tile that holds them has
+        // written model rows there too: every column is rewritten here, after all waves are done.
+        __syncthreads();
         const int q4 = lane & 3, k = wave * 16 + (lane >> 2);
         double s0 = 0, s1 = 0, s2 = 0;
         if (k <= nc) {
@@ -376,6 +391,7 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
     const int mfit = P.mfit, c00col = P.c00col;
     const int *unit_rows = P.unit_rows;
     const bool incol = c <= mfit, zero_col = c == c00col, res_col = c == mfit;
+    const int cl = min(c, mfit);  // the column a lane reads (lanes past mfit read a valid one, unused)
     // the summed column, clamped: lanes past mfit add a valid column and their sums are not used
     const int scol = min(16 * wave + pl, mfit);
     const double ym_a = sh.u.rows[c * kJP + mfit], ym_b = sh.u.rows[(64 + c) * kJP + mfit];
@@ -399,17 +415,20 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
         const double rave = 1 / ave;
         const double coef_a = rd.sl_a / ave, coef_b = rd.sl_b / ave;
         const double res_a = rd.y_a - coef_a * ym_a, res_b = rd.y_b - coef_b * ym_b;
-#pragma unroll 4
-        for (int r = r0 + wave; r < r1; r += kWaves2) {
-            const bool lo = r < 64;
-            const double coef = read_lane(lo ? coef_a : coef_b, r & 63);
-            const double ym = read_lane(lo ? ym_a : ym_b, r & 63);
-            const double res = read_lane(lo ? res_a : res_b, r & 63);
+        // row r's factors from lane r (rows below 64) or r - 64: the two halves as two loops, so the
+        // lane reads need no selects
+        auto update = [&](int r, double coef, double ym, double res) {
             double *row = sh.u.rows + r * kJP;
-            const double x = incol ? row[c] : 0.0;
+            const double x = row[cl];
             const double v = res_col ? res : zero_col ? 0.0 : coef * (x - (ym * s) * rave);
             if (incol) row[c] = v;
-        }
+        };
+        int r = r0 + wave;
+        const int rm = min(r1, 64);
+#pragma unroll 2
+        for (; r < rm; r += kWaves2) update(r, read_lane(coef_a, r), read_lane(ym_a, r), read_lane(res_a, r));
+#pragma unroll 2
+        for (; r < r1; r += kWaves2) update(r, read_lane(coef_b, r - 64), read_lane(ym_b, r - 64), read_lane(res_b, r - 64));
     }
 }
 
@@ -609,6 +628,7 @@ template <int NFM>
 __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
     const int tid = threadIdx.x, l = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    n = __builtin_amdgcn_readfirstlane(n);
     double *a = sh.u.g.cov, *b = sh.da;
     double row[kGJRows2];
     PivotCand c{-1.0, -1};
@@ -633,9 +653,11 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
         for (int st = 1; st < kWaves2; st *= 2)
 #pragma unroll
             for (int w = 0; w < kWaves2; w += 2 * st) pivot_merge(cw[w], cw[w + st]);
-        c = cw[0];
-        if (c.q < 0) return 1;
-        const int irow = c.q / n, icol = c.q - irow * n;
+        // every lane holds the same merged candidate: the pivot's row and column are wave-uniform, and
+        // so is every per-row decision below
+        const int cq = __builtin_amdgcn_readfirstlane(cw[0].q);
+        if (cq < 0) return 1;
+        const int irow = cq / n, icol = cq - irow * n;
         done |= 1ull << icol;
         if (wave == irow % kWaves2) {
             double v = 0;
@@ -655,25 +677,26 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
             if (l <= n) sh.u.g.srow[l] = v;
         }
         __syncthreads();
-        if (sh.u.g.piv_s == 0.0) return 2;
+        if (uniform_double(sh.u.g.piv_s) == 0.0) return 2;
         const double pr = (l <= n) ? sh.u.g.prow[l] : 0.0;
         const double moved = (irow != icol && l <= n) ? sh.u.g.srow[l] : 0.0;
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
+        const bool piv_lane = l == icol;
+        // lanes past n carry no matrix entries: whatever they compute is never read
 #pragma unroll
         for (int k = 0; k < kGJRows2; ++k) {
             const int ll = wave + kWaves2 * k;
+            if (ll >= n) continue;
+            if (ll == icol) {
+                row[k] = pr;
+                continue;
+            }
             const double cur = (ll == irow) ? moved : row[k];
             const double dum = read_lane(cur, icol);
-            const double v = ((l == icol) ? 0.0 : cur) - pr * dum;
-            if (ll < n) {
-                if (ll == icol) {
-                    row[k] = pr;
-                } else {
-                    row[k] = (l <= n) ? v : 0.0;
-                    if (cand_col && !((done >> ll) & 1)) pivot_take(c, v, ll * n + l);
-                }
-            }
+            const double v = (piv_lane ? 0.0 : cur) - pr * dum;
+            row[k] = v;
+            if (!((done >> ll) & 1) && cand_col) pivot_take(c, v, ll * n + l);
         }
     }
 #pragma unroll

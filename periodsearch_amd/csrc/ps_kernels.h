@@ -896,23 +896,40 @@ __device__ inline double swap_half(double x, bool rows32, int k) {  // element k
     return __longlong_as_double((long long)((unsigned long long)h2[k] << 32 | l2[k]));
 }
 
+// The larger of lanes l and l ^ 16 (rows32: l ^ 32), the same bits in both lanes.
+__device__ inline double pair_max(double x, bool rows32) {
+    return fmax(swap_half(x, rows32, 0), swap_half(x, rows32, 1));
+}
+__device__ inline int pair_max(int x, bool rows32) {
+    const unsigned u = unsigned(x);
+    auto r = rows32 ? __builtin_amdgcn_permlane32_swap(u, u, false, false) : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return max(int(r[0]), int(r[1]));
+}
+template <int CTRL>
+__device__ inline int dpp_i(int x) {
+    return int(__builtin_amdgcn_mov_dpp(unsigned(x), CTRL, 0xF, 0xF, false));
+}
+
 // Wave-wide argmax of the pivot candidates (a total order: larger |a|, then larger index), on DPP and
-// permlane swaps; every lane ends with the same candidate.
+// permlane swaps; every lane ends with the same candidate.  Two passes: the largest |a|, then the
+// largest index among the lanes that hold it.  A NaN |a| is never the maximum (fmax returns the other
+// operand), as a NaN never passes gauss_errc's |a| >= big; all NaN leaves no candidate.
 __device__ inline PivotCand pivot_wave_max(PivotCand c) {
-    pivot_merge(c, pivot_dpp<0xB1>(c));
-    pivot_merge(c, pivot_dpp<0x4E>(c));
-    pivot_merge(c, pivot_dpp<0x141>(c));
-    pivot_merge(c, pivot_dpp<0x140>(c));
-#pragma unroll
-    for (int r32 = 0; r32 < 2; ++r32) {
-        const unsigned q = unsigned(c.q);
-        auto qq = r32 ? __builtin_amdgcn_permlane32_swap(q, q, false, false) : __builtin_amdgcn_permlane16_swap(q, q, false, false);
-        PivotCand a{swap_half(c.v, r32, 0), int(qq[0])};
-        const PivotCand b{swap_half(c.v, r32, 1), int(qq[1])};
-        pivot_merge(a, b);
-        c = a;
-    }
-    return c;
+    double m = c.v;
+    m = fmax(m, dpp<0xB1>(m));
+    m = fmax(m, dpp<0x4E>(m));
+    m = fmax(m, dpp<0x141>(m));
+    m = fmax(m, dpp<0x140>(m));
+    m = pair_max(m, false);
+    m = pair_max(m, true);
+    int q = (c.v == m) ? c.q : -1;
+    q = max(q, dpp_i<0xB1>(q));
+    q = max(q, dpp_i<0x4E>(q));
+    q = max(q, dpp_i<0x141>(q));
+    q = max(q, dpp_i<0x140>(q));
+    q = pair_max(q, false);
+    q = pair_max(q, true);
+    return {q < 0 ? -1.0 : m, q};  // no candidate (or only NaN): {-1, -1}, as a lane starts
 }
 
 // A double every lane holds (an LDS broadcast), made provably wave-uniform: the LM control flow that
@@ -935,6 +952,7 @@ constexpr int kGJRows = (kMaxFit + kWaves - 1) / kWaves;  // rows held per wave
 __device__ inline int gauss_jordan(Shared &sh, int n) {
     const int tid = threadIdx.x, l = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform row ownership
+    n = __builtin_amdgcn_readfirstlane(n);
     double *a = sh.u.g.cov, *b = sh.da;
     double row[kGJRows];
     PivotCand c{-1.0, -1};
@@ -962,9 +980,11 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         for (int st = 1; st < kWaves; st *= 2)
 #pragma unroll
             for (int w = 0; w < kWaves; w += 2 * st) pivot_merge(cw[w], cw[w + st]);
-        c = cw[0];
-        if (c.q < 0) return 1;
-        const int irow = c.q / n, icol = c.q - irow * n;
+        // every lane holds the same candidate: the pivot's row and column are wave-uniform, and so is
+        // every per-row decision below
+        const int cq = __builtin_amdgcn_readfirstlane(cw[0].q);
+        if (cq < 0) return 1;
+        const int irow = cq / n, icol = cq - irow * n;
         if (tid == 0) {
             sh.indxr[i] = irow;
             sh.indxc[i] = icol;
@@ -990,24 +1010,26 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
             if (l <= n) sh.srow[l] = v;
         }
         __syncthreads();
-        if (sh.piv_s == 0.0) return 2;  // the same in every wave
+        if (uniform_double(sh.piv_s) == 0.0) return 2;  // the same in every wave
         const double pr = (l <= n) ? sh.prow[l] : 0.0;
         const double moved = (irow != icol && l <= n) ? sh.srow[l] : 0.0;
         // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
+        const bool piv_lane = l == icol;
+        // lanes past n carry no matrix entries: whatever they compute is never read
 #pragma unroll
         for (int k = 0; k < kGJRows; ++k) {
             const int ll = wave + kWaves * k;
-            const double cur = (ll == irow) ? moved : row[k];  // after the swap
-            const double dum = read_lane(cur, icol);            // the pivot-column entry of row ll
-            const double v = ((l == icol) ? 0.0 : cur) - pr * dum;
             if (ll < n) {
                 if (ll == icol) {
                     row[k] = pr;
                 } else {
-                    row[k] = (l <= n) ? v : 0.0;
-                    if (cand_col && !((done >> ll) & 1)) pivot_take(c, v, ll * n + l);
+                    const double cur = (ll == irow) ? moved : row[k];  // after the swap
+                    const double dum = read_lane(cur, icol);            // the pivot-column entry of row ll
+                    const double v = (piv_lane ? 0.0 : cur) - pr * dum;
+                    row[k] = v;
+                    if (!((done >> ll) & 1) && cand_col) pivot_take(c, v, ll * n + l);
                 }
             }
         }
