@@ -49,11 +49,10 @@ struct Shared2 {
     union {
         double rows[kPassRows * kJP];  // the pass's Jacobian rows (fitted columns + residual)
         struct {
-            double cov[64 * 64];       // Gauss-Jordan: the damped matrix
-            double prow[64], srow[64];
-            double red_v[kWaves2];
-            int red_q[kWaves2];
-            double piv_s;
+            double cov[64 * 64];             // Gauss-Jordan: the damped matrix
+            double cand[2][kWaves2][64];     // each wave's pivot candidate row (two pivots in flight)
+            double red_v[2][kWaves2];        // each wave's candidate |a| and index
+            int red_q[2][kWaves2];
         } g;
     } u;
 };
@@ -620,9 +619,18 @@ __device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Full-pivot Gauss-Jordan (gauss_errc.c:9-73) on sh.u.g.cov with right-hand side sh.da: the
-// algorithm of gauss_jordan (ps_kernels.h) for four waves, each holding rows w, w + 4, ... in
-// registers.  Returns 0, 1 (no valid pivot) or 2 (zero pivot).  Only the solution is written back.
+// Full-pivot Gauss-Jordan (gauss_errc.c:9-73) on sh.u.g.cov with right-hand side sh.da, four waves each
+// holding rows w, w + 4, ... in registers.  Returns 0, 1 (no valid pivot) or 2 (zero pivot); only the
+// solution is written back.
+//
+// The reference swaps the pivot row into row icol (:38-42).  Here rows stay in place and carry a label,
+// the index the row has in the reference's swapped matrix: the pivot row takes label icol and the row
+// labelled icol takes label irow, so every row receives the reference's arithmetic, candidates are
+// ordered by (|a|, label * n + column) exactly as the reference scans them (rows ascending, `>=`), and
+// the solution is read back by label.  Each wave publishes its best candidate together with the row
+// that holds it, so one barrier per pivot suffices: after it every wave knows the pivot and reads the
+// pivot row from the winner's slot.  The slots are double-buffered, since a fast wave publishes the
+// next pivot's candidate while a slow one may still read this pivot's row.
 // ---------------------------------------------------------------------------------------------
 template <int NFM>
 __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
@@ -631,79 +639,78 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
     n = __builtin_amdgcn_readfirstlane(n);
     double *a = sh.u.g.cov, *b = sh.da;
     double row[kGJRows2];
+    int lab[kGJRows2];  // wave-uniform
     PivotCand c{-1.0, -1};
 #pragma unroll
     for (int k = 0; k < kGJRows2; ++k) {
         const int ll = wave + kWaves2 * k;
+        lab[k] = ll;
         row[k] = (ll < n && l <= n) ? ((l < n) ? a[ll * n + l] : b[ll]) : 0.0;
         if (ll < n && l < n) pivot_take(c, row[k], ll * n + l);
     }
     unsigned long long done = 0;
     for (int i = 0; i < n; ++i) {
+        const int buf = i & 1;
         c = pivot_wave_max(c);
+        const int wq = __builtin_amdgcn_readfirstlane(c.q);
         if (l == 0) {
-            sh.u.g.red_v[wave] = c.v;
-            sh.u.g.red_q[wave] = c.q;
+            sh.u.g.red_v[buf][wave] = c.v;
+            sh.u.g.red_q[buf][wave] = wq;
+        }
+        if (wq >= 0) {
+            const int vr = wq / n;
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < kGJRows2; ++k)
+                if (lab[k] == vr) v = row[k];
+            sh.u.g.cand[buf][wave][l] = v;
         }
         __syncthreads();
         PivotCand cw[kWaves2];
 #pragma unroll
-        for (int w = 0; w < kWaves2; ++w) cw[w] = {sh.u.g.red_v[w], sh.u.g.red_q[w]};
+        for (int w = 0; w < kWaves2; ++w) cw[w] = {sh.u.g.red_v[buf][w], sh.u.g.red_q[buf][w]};
+        int ww = 0;  // the winning wave: candidate indices of different waves are distinct
 #pragma unroll
         for (int st = 1; st < kWaves2; st *= 2)
 #pragma unroll
             for (int w = 0; w < kWaves2; w += 2 * st) pivot_merge(cw[w], cw[w + st]);
-        // every lane holds the same merged candidate: the pivot's row and column are wave-uniform, and
-        // so is every per-row decision below
         const int cq = __builtin_amdgcn_readfirstlane(cw[0].q);
         if (cq < 0) return 1;
+#pragma unroll
+        for (int w = 1; w < kWaves2; ++w)
+            if (__builtin_amdgcn_readfirstlane(sh.u.g.red_q[buf][w]) == cq) ww = w;
         const int irow = cq / n, icol = cq - irow * n;
         done |= 1ull << icol;
-        if (wave == irow % kWaves2) {
-            double v = 0;
-#pragma unroll
-            for (int k = 0; k < kGJRows2; ++k)
-                if (k == irow / kWaves2) v = row[k];
-            const double s = read_lane(v, icol);
-            const double pivinv = 1.0 / s;
-            if (l <= n) sh.u.g.prow[l] = ((l == icol) ? 1.0 : v) * pivinv;
-            if (l == 0) sh.u.g.piv_s = s;
-        }
-        if (irow != icol && wave == icol % kWaves2) {
-            double v = 0;
-#pragma unroll
-            for (int k = 0; k < kGJRows2; ++k)
-                if (k == icol / kWaves2) v = row[k];
-            if (l <= n) sh.u.g.srow[l] = v;
-        }
-        __syncthreads();
-        if (uniform_double(sh.u.g.piv_s) == 0.0) return 2;
-        const double pr = (l <= n) ? sh.u.g.prow[l] : 0.0;
-        const double moved = (irow != icol && l <= n) ? sh.u.g.srow[l] : 0.0;
+        const double *prow = sh.u.g.cand[buf][ww];
+        const double s = uniform_double(prow[icol]);
+        if (s == 0.0) return 2;
+        const double pivinv = 1.0 / s;
+        // the new pivot row, gauss_errc.c:44-53 (lanes past n are not read)
+        const double pr = ((l == icol) ? 1.0 : prow[l]) * pivinv;
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
         const bool piv_lane = l == icol;
-        // lanes past n carry no matrix entries: whatever they compute is never read
+        // eliminate column icol from every other row (:55-62), searching the next pivot
 #pragma unroll
         for (int k = 0; k < kGJRows2; ++k) {
-            const int ll = wave + kWaves2 * k;
-            if (ll >= n) continue;
-            if (ll == icol) {
+            const int lb = lab[k];
+            if (lb >= n) continue;
+            if (lb == irow) {
                 row[k] = pr;
+                lab[k] = icol;
                 continue;
             }
-            const double cur = (ll == irow) ? moved : row[k];
+            if (lb == icol) lab[k] = irow;
+            const double cur = row[k];
             const double dum = read_lane(cur, icol);
             const double v = (piv_lane ? 0.0 : cur) - pr * dum;
             row[k] = v;
-            if (!((done >> ll) & 1) && cand_col) pivot_take(c, v, ll * n + l);
+            if (!((done >> lab[k]) & 1) && cand_col) pivot_merge(c, {fabs(v), lab[k] * n + l});
         }
     }
 #pragma unroll
-    for (int k = 0; k < kGJRows2; ++k) {
-        const int ll = wave + kWaves2 * k;
-        if (ll < n && l == n) b[ll] = row[k];
-    }
+    for (int k = 0; k < kGJRows2; ++k)
+        if (lab[k] < n && l == n) b[lab[k]] = row[k];
     return 0;
 }
 
