@@ -97,11 +97,7 @@ struct Shared {
     float nxf[kMaxFac], nyf[kMaxFac], nzf[kMaxFac];     // FP32 copies (mixed precision)
     float areaf[kMaxFac], aclf[kMaxFac], aclsf[kMaxFac];
     double da[64], col[64], b0[64];
-    double prow[64], srow[64];  // Gauss-Jordan: the new pivot row, the row moved by a swap
-    int indxr[64], indxc[64];
-    double red_v[kWaves];   // Gauss-Jordan: each wave's pivot candidate |a| and index
-    int red_q[kWaves];
-    double piv_s;           // the pivot value, from the pivot row's owner
+    int indxr[64], indxc[64];  // Gauss-Jordan: the pivots' rows and columns (for the refinement)
     int ipiv[64];
     double scal[8];
     int iscal[4];
@@ -114,6 +110,9 @@ struct Shared {
         struct {
             double cov[64 * 64];     // Gauss-Jordan: the damped matrix, then its inverse
             double cov0[64 * 64];    // the damped matrix kept for iterative refinement
+            double cand[2][kWaves][64];  // each wave's pivot candidate row (two pivots in flight)
+            double red_v[2][kWaves];     // each wave's candidate |a| and index
+            int red_q[2][kWaves];
         } g;
         struct {
             double red[64 * 64];     // phase D: cross-wave sum of the normal equations
@@ -861,11 +860,15 @@ __device__ __forceinline__ double mrqcof(const DevProblem &P, double *J, double 
 // Pivot: the largest |a| over unpivoted rows and columns, ties to the last one in row-major order
 // (the reference's `>=` scan); NaN entries are never chosen.  Returns 0, 1 (no valid pivot) or
 // 2 (zero pivot).  Wave w keeps rows w, w + 8, ... in registers for the whole solve, lane l holding
-// column l (column n = the right-hand side).  Per pivot only the new pivot row and, for a row swap,
-// the row moved into the pivot row's place pass through LDS; the pivot-column entry of a row is read
-// from its lane by a shuffle; the search for the next pivot runs inside the elimination.  Two barriers
-// per pivot.  On success the inverse (columns not unscrambled, gauss_errc.c:63-67) and the solution
-// are written back to sh.u.cov / sh.da.
+// column l (column n = the right-hand side).  The reference swaps the pivot row into row icol
+// (:38-42); here rows stay in place and carry a label, the index the row has in the reference's
+// swapped matrix (the pivot row takes label icol, the row labelled icol takes label irow), so every
+// row receives the reference's arithmetic and candidates are ordered as the reference scans them.
+// Each wave publishes its best candidate with the row that holds it (double-buffered LDS slots): one
+// barrier per pivot, after which every wave reads the pivot row from the winner's slot.  The
+// pivot-column entry of a row is read from its lane; the search for the next pivot runs inside the
+// elimination.  On success the inverse (columns not unscrambled, gauss_errc.c:63-67) and the solution
+// are written back to sh.u.cov / sh.da by label.
 // ---------------------------------------------------------------------------------------------
 struct PivotCand {
     double v;  // |a|
@@ -955,93 +958,90 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
     n = __builtin_amdgcn_readfirstlane(n);
     double *a = sh.u.g.cov, *b = sh.da;
     double row[kGJRows];
+    int lab[kGJRows];  // wave-uniform
     PivotCand c{-1.0, -1};
 #pragma unroll
     for (int k = 0; k < kGJRows; ++k) {
         const int ll = wave + kWaves * k;
+        lab[k] = ll;
         row[k] = (ll < n && l <= n) ? ((l < n) ? a[ll * n + l] : b[ll]) : 0.0;
         if (ll < n && l < n) pivot_take(c, row[k], ll * n + l);
     }
-    unsigned long long done = 0;  // pivoted rows / columns
+    unsigned long long done = 0;  // pivoted columns (= labels of pivot rows)
     for (int i = 0; i < n; ++i) {
+        const int buf = i & 1;
         // workgroup argmax: each wave on the VALU, then every lane combines the eight wave results
         c = pivot_wave_max(c);
+        const int wq = __builtin_amdgcn_readfirstlane(c.q);
         if (l == 0) {
-            sh.red_v[wave] = c.v;
-            sh.red_q[wave] = c.q;
+            sh.u.g.red_v[buf][wave] = c.v;
+            sh.u.g.red_q[buf][wave] = wq;
+        }
+        if (wq >= 0) {
+            const int vr = wq / n;
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < kGJRows; ++k)
+                if (lab[k] == vr) v = row[k];
+            sh.u.g.cand[buf][wave][l] = v;
         }
         __syncthreads();
         // the workgroup's candidate: a tree over the eight wave candidates (the order is total, so the
-        // result does not depend on the grouping)
+        // result does not depend on the grouping); candidate indices of different waves are distinct
         PivotCand cw[kWaves];
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) cw[w] = {sh.red_v[w], sh.red_q[w]};
+        for (int w = 0; w < kWaves; ++w) cw[w] = {sh.u.g.red_v[buf][w], sh.u.g.red_q[buf][w]};
 #pragma unroll
         for (int st = 1; st < kWaves; st *= 2)
 #pragma unroll
             for (int w = 0; w < kWaves; w += 2 * st) pivot_merge(cw[w], cw[w + st]);
-        // every lane holds the same candidate: the pivot's row and column are wave-uniform, and so is
-        // every per-row decision below
         const int cq = __builtin_amdgcn_readfirstlane(cw[0].q);
         if (cq < 0) return 1;
+        int ww = 0;
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w)
+            if (__builtin_amdgcn_readfirstlane(sh.u.g.red_q[buf][w]) == cq) ww = w;
         const int irow = cq / n, icol = cq - irow * n;
         if (tid == 0) {
             sh.indxr[i] = irow;
             sh.indxc[i] = icol;
         }
         done |= 1ull << icol;
-        // publish the new pivot row (old row irow scaled, gauss_errc.c:44-53) with the pivot value and,
-        // for a swap, old row icol, which moves to row irow (:38-42)
-        if (wave == irow % kWaves) {
-            double v = 0;
-#pragma unroll
-            for (int k = 0; k < kGJRows; ++k)
-                if (k == irow / kWaves) v = row[k];
-            const double s = read_lane(v, icol);
-            const double pivinv = 1.0 / s;
-            if (l <= n) sh.prow[l] = ((l == icol) ? 1.0 : v) * pivinv;
-            if (l == 0) sh.piv_s = s;
-        }
-        if (irow != icol && wave == icol % kWaves) {
-            double v = 0;
-#pragma unroll
-            for (int k = 0; k < kGJRows; ++k)
-                if (k == icol / kWaves) v = row[k];
-            if (l <= n) sh.srow[l] = v;
-        }
-        __syncthreads();
-        if (uniform_double(sh.piv_s) == 0.0) return 2;  // the same in every wave
-        const double pr = (l <= n) ? sh.prow[l] : 0.0;
-        const double moved = (irow != icol && l <= n) ? sh.srow[l] : 0.0;
+        const double *prow = sh.u.g.cand[buf][ww];
+        const double s = uniform_double(prow[icol]);
+        if (s == 0.0) return 2;  // the same in every wave
+        const double pivinv = 1.0 / s;
+        // the new pivot row (gauss_errc.c:44-53; lanes past n are not read)
+        const double pr = ((l == icol) ? 1.0 : prow[l]) * pivinv;
         // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
         const bool piv_lane = l == icol;
-        // lanes past n carry no matrix entries: whatever they compute is never read
 #pragma unroll
         for (int k = 0; k < kGJRows; ++k) {
-            const int ll = wave + kWaves * k;
-            if (ll < n) {
-                if (ll == icol) {
-                    row[k] = pr;
-                } else {
-                    const double cur = (ll == irow) ? moved : row[k];  // after the swap
-                    const double dum = read_lane(cur, icol);            // the pivot-column entry of row ll
-                    const double v = (piv_lane ? 0.0 : cur) - pr * dum;
-                    row[k] = v;
-                    if (!((done >> ll) & 1) && cand_col) pivot_take(c, v, ll * n + l);
-                }
+            const int lb = lab[k];
+            if (lb >= n) continue;
+            if (lb == irow) {
+                row[k] = pr;
+                lab[k] = icol;
+                continue;
             }
+            if (lb == icol) lab[k] = irow;                  // the row the swap moves (:38-42)
+            const double cur = row[k];
+            const double dum = read_lane(cur, icol);        // the pivot-column entry of the row
+            const double v = (piv_lane ? 0.0 : cur) - pr * dum;
+            row[k] = v;
+            if (!((done >> lab[k]) & 1) && cand_col) pivot_merge(c, {fabs(v), lab[k] * n + l});
         }
     }
 #pragma unroll
     for (int k = 0; k < kGJRows; ++k) {
-        const int ll = wave + kWaves * k;
-        if (ll < n && l <= n) {
+        const int lb = lab[k];
+        if (lb < n && l <= n) {
             if (l < n)
-                a[ll * n + l] = row[k];
+                a[lb * n + l] = row[k];
             else
-                b[ll] = row[k];
+                b[lb] = row[k];
         }
     }
     return 0;
