@@ -121,8 +121,8 @@ __device__ inline double facet_sum(const double *area, const double *v, int nf) 
 // raw rows, all ma columns, in input order.
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool LS, int NFM>
-__device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &sh, int tile0, int ntl, bool last,
-                                            const double *area, double *raw_y, double *raw_dyda) {
+__device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &sh, int tile0, int ntl, int rot,
+                                            bool last, const double *area, double *raw_y, double *raw_dyda) {
     using Pr = Prec<false>;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int pl = lane & 15, kg = lane >> 4;
@@ -131,7 +131,9 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
     const double cl_d = exp(a[nc + 6]);
     const double p1 = a[nc + 3], p2 = a[nc + 4];
     const int nchunk = P.nf >> 2;
-    const int tl = 2 * wave;  // the wave's first tile, pass-relative
+    // the wave's first tile, pass-relative: waves take tile pairs in an order that rotates with the
+    // pass, so the short pair of a pass with an odd tile count falls on each SIMD in turn
+    const int tl = 2 * ((wave + rot) & 3);
     // the row column of this lane's shape column of each MFMA tile; a coefficient that is not fitted
     // (or past nc) goes to column mfit, which the same wave rewrites afterwards with the model value
     // or residual of the point (its LDS writes are done in program order)
@@ -501,41 +503,40 @@ __device__ __forceinline__ void gram_pass_w(const DevProblem &P, const Shared2<N
 
 template <int NB, int NFM>
 __device__ __forceinline__ void gram_pass_nb(const DevProblem &P, const Shared2<NFM> &sh, const PassRowData &rd, int ntl,
-                                             Prec<false>::V4 acc[kOwn]) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wave == 0)
+                                             int ow, Prec<false>::V4 acc[kOwn]) {
+    if (ow == 0)
         gram_pass_w<NB, 0>(P, sh, rd, ntl, acc);
-    else if (wave == 1)
+    else if (ow == 1)
         gram_pass_w<NB, 1>(P, sh, rd, ntl, acc);
-    else if (wave == 2)
+    else if (ow == 2)
         gram_pass_w<NB, 2>(P, sh, rd, ntl, acc);
     else
         gram_pass_w<NB, 3>(P, sh, rd, ntl, acc);
 }
 
-// nb = the number of 16-column blocks covering mfit + 1 columns (wave-uniform)
+// nb = the number of 16-column blocks covering mfit + 1 columns, ow = the ownership set the wave
+// holds in this pass (both wave-uniform)
 template <int NFM>
 __device__ __forceinline__ void gram_pass(int nb, const DevProblem &P, const Shared2<NFM> &sh, const PassRowData &rd,
-                                          int ntl, Prec<false>::V4 acc[kOwn]) {
+                                          int ntl, int ow, Prec<false>::V4 acc[kOwn]) {
     if (nb == 4)
-        gram_pass_nb<4>(P, sh, rd, ntl, acc);
+        gram_pass_nb<4>(P, sh, rd, ntl, ow, acc);
     else if (nb == 3)
-        gram_pass_nb<3>(P, sh, rd, ntl, acc);
+        gram_pass_nb<3>(P, sh, rd, ntl, ow, acc);
     else if (nb == 2)
-        gram_pass_nb<2>(P, sh, rd, ntl, acc);
+        gram_pass_nb<2>(P, sh, rd, ntl, ow, acc);
     else
-        gram_pass_nb<1>(P, sh, rd, ntl, acc);
+        gram_pass_nb<1>(P, sh, rd, ntl, ow, acc);
 }
 
 // The owned blocks into alpha (full, the lower triangle mirrored as mrqcof.c:143-145 does), beta and
 // (into sh.scal[0]) chi2.
 template <int NFM>
-__device__ __forceinline__ void gram_store(int nb, const DevProblem &P, Shared2<NFM> &sh,
+__device__ __forceinline__ void gram_store(int nb, const DevProblem &P, Shared2<NFM> &sh, int ow,
                                            const Prec<false>::V4 acc[kOwn], double *alpha, double *beta) {
     const int lane = threadIdx.x & 63, pl = lane & 15, kg = lane >> 4;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mfit = P.mfit;
-    const OwnList o = own_blocks(nb, wave);
+    const OwnList o = own_blocks(nb, ow);
 #pragma unroll
     for (int b = 0; b < kOwn; ++b) {
         if (b < o.n) {
@@ -558,11 +559,12 @@ __device__ __forceinline__ void gram_store(int nb, const DevProblem &P, Shared2<
     }
 }
 
-// The owned accumulators to / from the slot's stash (between passes).
-__device__ __forceinline__ void stash_io(int nb, Prec<false>::V4 acc[kOwn], double *stash, bool store) {
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int n = own_blocks(nb, wave).n;
-    d4 *p = reinterpret_cast<d4 *>(stash) + size_t(wave) * kOwn * 64 + lane;
+// The accumulators of ownership set ow to / from the slot's stash (between passes; the set changes
+// hands between passes, the barrier at the end of each pass orders the store and the load).
+__device__ __forceinline__ void stash_io(int nb, int ow, Prec<false>::V4 acc[kOwn], double *stash, bool store) {
+    const int lane = threadIdx.x & 63;
+    const int n = own_blocks(nb, ow).n;
+    d4 *p = reinterpret_cast<d4 *>(stash) + size_t(ow) * kOwn * 64 + lane;
 #pragma unroll
     for (int b = 0; b < kOwn; ++b) {
         if (b < n) {
@@ -597,22 +599,25 @@ __device__ __forceinline__ double mrqcof2(const DevProblem &P, Shared2<NFM> &sh,
         // latency-bound phase (lightcurve sums, Gram, Gauss-Jordan), its instructions go first and the
         // facet loop fills the gaps
         __builtin_amdgcn_s_setprio(0);
-        model_rows2<NT, R, LS>(P, sh, tile0, ntl, last, area, raw_y, raw_dyda);
+        model_rows2<NT, R, LS>(P, sh, tile0, ntl, k & 3, last, area, raw_y, raw_dyda);
         __builtin_amdgcn_s_setprio(2);
         const PassRowData rd = load_pass_rows(P, tile0, wqc);
         __syncthreads();
         normalise_pass(P, sh, rd, u0, u1);
         __syncthreads();
-        // the accumulators live only through phase D: parked in the slot between passes
+        // the accumulators live only through phase D: parked in the slot between passes.  The
+        // ownership sets with three blocks go to waves 0, 1 in even passes and 2, 3 in odd ones, so the
+        // Gram work falls evenly on the four SIMDs
+        const int ow = (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + 2 * (k & 1)) & 3;
         Pd::V4 acc[kOwn];
 #pragma unroll
         for (int b = 0; b < kOwn; ++b) acc[b] = Pd::zero();
-        if (k > 0) stash_io(nb, acc, stash, false);
-        gram_pass(nb, P, sh, rd, ntl, acc);
+        if (k > 0) stash_io(nb, ow, acc, stash, false);
+        gram_pass(nb, P, sh, rd, ntl, ow, acc);
         if (last)
-            gram_store(nb, P, sh, acc, alpha, beta);
+            gram_store(nb, P, sh, ow, acc, alpha, beta);
         else
-            stash_io(nb, acc, stash, true);
+            stash_io(nb, ow, acc, stash, true);
         __syncthreads();
     }
     return sh.scal[0];
