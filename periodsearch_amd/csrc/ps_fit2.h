@@ -168,33 +168,35 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
             ts[h].d1 = ts[h].d2 = 0;
             ts[h].br = 0;
         }
-        // Dsph through L2, one chunk ahead: the loop runs two chunks per trip with two operand sets, so
-        // each chunk's loads are issued a whole chunk of work before their use (the table has a zero
-        // chunk of facets past its end, and nchunk = nf / 4 is even).  Buffer loads: their results
-        // cannot be folded back into the iteration that uses them, the chunk offset goes in a scalar,
-        // and the record count bounds every access.
-        const __amdgpu_buffer_rsrc_t dsr = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<double *>(P.dsph), short(0), (P.nf + 4) * pitch * int(sizeof(double)), 0x00020000);
-        const int vb = (kg * pitch + pl) * int(sizeof(double));       // the lane's operand in a chunk
-        const int vr = (kg * pitch + 16 * NT) * int(sizeof(double));  // the leftover columns
-        const int cbytes = 4 * pitch * int(sizeof(double));
+        // Dsph through L2: the operands of chunk c + 1 are loaded while chunk c is computed (the table
+        // has a zero chunk of facets past its end, so the last chunk's load needs no branch)
+        const __attribute__((address_space(1))) double *dpl =
+            reinterpret_cast<const __attribute__((address_space(1))) double *>(
+                reinterpret_cast<unsigned long long>(P.dsph)) + kg * pitch + pl;
+        const __attribute__((address_space(1))) double *dpr = dpl - pl + 16 * NT;
+        const int dstep = 4 * pitch;
         constexpr int RP = R <= 1 ? R : 0;  // leftover columns prefetched as well
-        auto ld = [&](int off) {
-            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(dsr, off, 0, 0));
-        };
-        auto load_ops = [&](int c, double (&Bx)[NT], double (&Brx)[RP > 0 ? RP : 1]) {
+        double Bn[NT], Brn[RP > 0 ? RP : 1];
 #pragma unroll
-            for (int tt = 0; tt < NT; ++tt) Bx[tt] = ld(vb + 128 * tt + c * cbytes);
+        for (int tt = 0; tt < NT; ++tt) Bn[tt] = dpl[16 * tt];
 #pragma unroll
-            for (int r = 0; r < RP; ++r) Brx[r] = ld(vr + 8 * r + c * cbytes);
-        };
-        auto chunk = [&](int c, const double (&Bx)[NT], const double (&Brx)[RP > 0 ? RP : 1]) {
+        for (int r = 0; r < RP; ++r) Brn[r] = dpr[r];
+#pragma unroll 2
+        for (int c = 0; c < nchunk; ++c) {
             const int j = 4 * c + kg;
             const double n0 = sh.nx[j], n1 = sh.ny[j], n2 = sh.nz[j], Ac = sh.acl[j], As = sh.acls[j];
             const double A = LS ? gload(area, j) : 0.0;
-            double Br[R > 0 ? R : 1];
+            double B[NT], Br[R > 0 ? R : 1];
 #pragma unroll
-            for (int r = 0; r < R; ++r) Br[r] = (r < RP) ? Brx[r < RP ? r : 0] : ld(vr + 8 * r + c * cbytes);
+            for (int tt = 0; tt < NT; ++tt) B[tt] = Bn[tt];
+#pragma unroll
+            for (int r = 0; r < R; ++r) Br[r] = (r < RP) ? Brn[r < RP ? r : 0] : dpr[r];
+            dpl += dstep;
+            dpr += dstep;
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt) Bn[tt] = dpl[16 * tt];
+#pragma unroll
+            for (int r = 0; r < RP; ++r) Brn[r] = dpr[r];
 #pragma unroll
             for (int h = 0; h < KT; ++h) {
                 TileSums<NT, R, false> &s = ts[h];
@@ -216,18 +218,10 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
                 s.S[0] += ad * n0; s.S[1] += ad * n1; s.S[2] += ad * n2;
                 s.U[0] += ad0 * n0; s.U[1] += ad0 * n1; s.U[2] += ad0 * n2;
 #pragma unroll
-                for (int tt = 0; tt < NT; ++tt) s.acc[tt] = Pr::mma(X, Bx[tt], s.acc[tt]);
+                for (int tt = 0; tt < NT; ++tt) s.acc[tt] = Pr::mma(X, B[tt], s.acc[tt]);
 #pragma unroll
                 for (int r = 0; r < R; ++r) s.racc[r] += X * Br[r];
             }
-        };
-        double BA[NT], BAr[RP > 0 ? RP : 1], BB[NT], BBr[RP > 0 ? RP : 1];
-        load_ops(0, BA, BAr);
-        for (int c = 0; c < nchunk; c += 2) {
-            load_ops(c + 1, BB, BBr);
-            chunk(c, BA, BAr);
-            load_ops(c + 2, BA, BAr);
-            chunk(c + 1, BB, BBr);
         }
         double scale[KT], dphp[KT][3];
 #pragma unroll
