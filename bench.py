@@ -239,12 +239,14 @@ def bench_batch(args):
     """The batched uneven workload: 512 seeded asteroids (synth.config4), a window of
     --batch-periods trial periods x 10 poles around each true period, all (asteroid, period, pole)
     fits as one ps_batch_run (a graph launch of persistent workgroups draining a device work queue).
-    A step replays the same batch; the flops count each fit's own iteration number."""
+    A step replays the same batch; the flops count each fit's own iteration number.  The same batch
+    then runs in mixed precision (the mixed queue kernel), reported beside the FP64 value with its
+    speedup and its agreement with the FP64 fits."""
     import numpy as np
 
-    from periodsearch_amd import Batch, PeriodSearch, synth
+    from periodsearch_amd import PS_FP64, PS_MIXED, Batch, PeriodSearch, synth
 
-    searches, conw, prob, freq, pole, f_eval = [], [], [], [], [], []
+    searches, conw, prob, freq, pole, f_shape, f_ne = [], [], [], [], [], [], []
     for i in range(synth.CONFIG4_N):
         text, truth = synth.generate(synth.config4(i))
         ps = PeriodSearch.from_text(text).prepare(device=0)
@@ -257,24 +259,34 @@ def bench_batch(args):
         pole.extend(np.tile(np.arange(1, 11), len(n)))
         prob.extend([i] * (10 * len(n)))
         fs, fn = flops_per_eval(info)
-        f_eval.append(fs + fn)
-    conw, prob, freq, pole, f_eval = map(np.asarray, (conw, prob, freq, pole, f_eval))
-    with Batch(searches) as b:
-        for _ in range(args.warmup):
-            b.run(conw, prob, freq, pole)
-        b.kernel_time(reset=True)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            res = b.run(conw, prob, freq, pole)
-        wall = time.perf_counter() - t0
-        kernel_ms, launches = b.kernel_time()
+        f_shape.append(fs)
+        f_ne.append(fn)
+    conw, prob, freq, pole, f_shape, f_ne = map(np.asarray, (conw, prob, freq, pole, f_shape, f_ne))
+
+    def run(precision):
+        for ps in searches:
+            ps.set_precision(precision)
+        with Batch(searches) as b:
+            for _ in range(args.warmup):
+                b.run(conw, prob, freq, pole)
+            b.kernel_time(reset=True)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                res = b.run(conw, prob, freq, pole)
+            wall = time.perf_counter() - t0
+            kernel_ms, launches = b.kernel_time()
+        return res, wall, kernel_ms * 1e-3 / launches
+
+    res, wall, avg_launch = run(PS_FP64)
+    mres, mwall, mavg = (None, None, None) if args.precision == "fp64" else run(PS_MIXED)
     for ps in searches:
         ps.close()
     nfits = len(freq)
-    flops = float(np.sum((res["niter"] + 1) * f_eval[prob]))
-    avg_launch = kernel_ms * 1e-3 / launches
+    fs_tot = float(np.sum((res["niter"] + 1) * f_shape[prob]))
+    fn_tot = float(np.sum((res["niter"] + 1) * f_ne[prob]))
+    flops = fs_tot + fn_tot
     achieved = flops / avg_launch / 1e12
-    return {
+    line = {
         "metric": METRIC_CONFIG4, "value": round(nfits * args.steps / wall, 2), "unit": "fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -289,6 +301,38 @@ def bench_batch(args):
                      "avg_launch_ms": round(avg_launch * 1e3, 3)},
         "cpu_baseline": None,
     }
+    if mres is not None:
+        # flops from the mixed run's own iteration counts; the peak weighs the FP32 (shape contraction)
+        # and FP64 (normal equations) parts by their flops
+        mfs = float(np.sum((mres["niter"] + 1) * f_shape[prob]))
+        mfn = float(np.sum((mres["niter"] + 1) * f_ne[prob]))
+        mpeak = (mfs + mfn) / (mfs / PEAK_FP32 + mfn / PEAK_FP64)
+        mach = (mfs + mfn) / mavg / 1e12
+        ok = np.isfinite(res["chisq"]) & np.isfinite(mres["chisq"])
+        err = np.abs(mres["chisq"][ok] - res["chisq"][ok]) / np.abs(res["chisq"][ok])
+        # the best trial period of each asteroid's window (the lowest rms over its poles), FP64 vs mixed
+        same = []
+        for i in range(synth.CONFIG4_N):
+            sel = np.flatnonzero(prob == i)
+            per = np.round(freq[sel], 15)
+            uniq = np.unique(per)
+            if len(uniq) < 2:
+                continue
+            best = lambda r: uniq[np.argmin([np.nanmin(r["dev"][sel][per == u]) for u in uniq])]
+            same.append(best(res) == best(mres))
+        mval = nfits * args.steps / mwall
+        line["mixed"] = {
+            "value": round(mval, 2), "unit": "fits/s", "ms_per_step": round(mwall / args.steps * 1e3, 3),
+            "speedup_vs_fp64": round(mval / line["value"], 3),
+            "roofline": {"bound": "mfma", "achieved": round(mach, 3), "peak": round(mpeak, 2), "unit": "TFLOP/s",
+                         "frac": round(mach / mpeak, 4), "traffic": None, "kernel": "legacy (mixed queue)",
+                         "flops_per_step": mfs + mfn, "avg_launch_ms": round(mavg * 1e3, 3)},
+            "agreement": {"asteroids": len(same), "same_best_period": int(np.sum(same)),
+                          "fit_chi2_rel_err_median": float(f"{np.median(err):.3g}"),
+                          "fit_chi2_rel_err_p90": float(f"{np.quantile(err, 0.9):.3g}"),
+                          "same_iteration_count": round(float(np.mean(mres["niter"] == res["niter"])), 3)},
+        }
+    return line
 
 
 def main():

@@ -951,6 +951,8 @@ struct ps_batch {
     double *d_conw = nullptr;
     int *d_counter = nullptr;
     Slots2 pool{};
+    bool mixed = false;  // mixed precision: queue_kernel_mixed over the legacy problem descriptions
+    Slots lpool{};       // its per-slot buffers
     int slots = 0;
     psk::QueueJob *d_jobs = nullptr;
     ps_fit_result *d_out = nullptr;
@@ -978,16 +980,24 @@ extern "C" void ps_batch_release(ps_batch *b) {
 extern "C" int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out) {
     if (!hs || nh < 1 || !out) return ps_fail(PS_ERR_ARG, "ps_batch_create: bad argument");
     *out = nullptr;
+    const bool mixed = hs[0] && hs[0]->precision == PS_MIXED;
     for (int i = 0; i < nh; ++i) {
         if (!hs[i]) return ps_fail(PS_ERR_ARG, "ps_batch_create: problem %d is null", i);
         if (hs[i]->device != hs[0]->device) return ps_fail(PS_ERR_ARG, "ps_batch_create: problems on different devices");
-        if (!v2_ok(hs[i]))
+        if ((hs[i]->precision == PS_MIXED) != mixed)
+            return ps_fail(PS_ERR_ARG, "ps_batch_create: problems %d and 0 have different precisions", i);
+        if (mixed) {
+            if (layout_id(hs[i]->var) > 3 || !hs[i]->var.lds_mix)
+                return ps_fail(PS_ERR_UNSUPPORTED, "ps_batch_create: problem %d: a mixed batch needs Lmax <= 6", i);
+        } else if (!v2_ok(hs[i])) {
             return ps_fail(PS_ERR_UNSUPPORTED, "ps_batch_create: problem %d needs the FP64 J-free kernels (precision FP64, "
                                                "every relative lightcurve <= %d points)", i, psk::kPassRows);
+        }
     }
     HIPCHECK(hipSetDevice(hs[0]->device));
     auto *b = new ps_batch();
     b->device = hs[0]->device;
+    b->mixed = mixed;
     b->hs.assign(hs, hs + nh);
     auto fail = [&](int rc) {
         ps_batch_release(b);
@@ -998,18 +1008,20 @@ extern "C" int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out) {
         return fail(ps_fail(PS_ERR_DEVICE, "stream/event creation failed"));
     std::vector<const DevProblem *> probs(nh);
     std::vector<int> layout(nh);
-    size_t amax = 1, bmax = 1, fmax = 1;
+    size_t amax = 1, bmax = 1, fmax = 1, nptmax = 1, segmax = 1;
     b->cost.resize(nh);
     for (int i = 0; i < nh; ++i) {
         const ps_handle *h = hs[i];
-        probs[i] = h->dP2;
+        probs[i] = mixed ? h->dP : h->dP2;
         layout[i] = layout_id(h->var);
         amax = std::max(amax, size_t(h->mfit) * h->mfit);
         bmax = std::max(bmax, size_t(h->mfit));
         fmax = std::max(fmax, size_t(h->nfac));
-        const double iters = h->P2.n_iter_max > 0 ? h->P2.n_iter_max : h->P2.n_iter_min;
-        b->cost[i] = (iters + 1) * 16.0 * h->P2.pass_tile0[h->P2.npass] *
-                     (2.0 * h->nfac * h->ncoef + double(h->mfit) * (h->mfit + 1));
+        nptmax = std::max(nptmax, size_t(h->npts));
+        segmax = std::max(segmax, size_t(h->nseg));
+        const double iters = h->P.n_iter_max > 0 ? h->P.n_iter_max : h->P.n_iter_min;
+        const double rows = mixed ? double(h->npts) : 16.0 * h->P2.pass_tile0[h->P2.npass];
+        b->cost[i] = (iters + 1) * rows * (2.0 * h->nfac * h->ncoef + double(h->mfit) * (h->mfit + 1));
     }
     auto &o = b->owned;
     b->d_probs = dev_upload(probs.data(), nh, o);
@@ -1023,8 +1035,34 @@ extern "C" int ps_batch_create(ps_handle *const *hs, int nh, ps_batch **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) return fail(ps_fail(PS_ERR_DEVICE, "device query failed"));
     int per_cu = 1;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::queue_kernel2<false>, psk::kWG2, 0);
+    if (mixed)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::queue_kernel_mixed<false>, kWG, 0);
+    else
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, psk::queue_kernel2<false>, psk::kWG2, 0);
     b->slots = std::max(1, per_cu) * prop.multiProcessorCount;
+    if (mixed) {
+        Slots &L = b->lpool;
+        L.j_stride = nptmax * psk::kMap;
+        L.y_stride = nptmax;
+        L.dave_stride = segmax * psk::kMap;
+        L.a_stride = amax;
+        L.b_stride = bmax;
+        const size_t n = size_t(b->slots);
+        double *pj = nullptr, *py = nullptr, *pd = nullptr, *pa = nullptr, *pb = nullptr;
+        if (hipMalloc(&pj, sizeof(double) * L.j_stride * n) != hipSuccess || (o.push_back(pj), false) ||
+            hipMalloc(&py, sizeof(double) * L.y_stride * n) != hipSuccess || (o.push_back(py), false) ||
+            hipMalloc(&pd, sizeof(double) * L.dave_stride * n) != hipSuccess || (o.push_back(pd), false) ||
+            hipMalloc(&pa, sizeof(double) * amax * 2 * n) != hipSuccess || (o.push_back(pa), false) ||
+            hipMalloc(&pb, sizeof(double) * bmax * 2 * n) != hipSuccess || (o.push_back(pb), false))
+            return fail(ps_fail(PS_ERR_NOMEM, "batch scratch allocation failed (%d slots)", b->slots));
+        L.J = pj;
+        L.ytemp = py;
+        L.dave = pd;
+        L.alpha = pa;
+        L.beta = pb;
+        *out = b;
+        return PS_OK;
+    }
     Slots2 &T = b->pool;
     T.a_stride = amax;
     T.b_stride = bmax;
@@ -1057,6 +1095,18 @@ extern "C" int ps_batch_run(ps_batch *b, const double *conw_r, int njobs, const 
     if (rc != PS_OK) return rc;
     if (njobs == 0) return PS_OK;
     HIPCHECK(hipSetDevice(b->device));
+    if (b->mixed) {
+        // the materialised-Jacobian kernels weight the convexity rows from each problem's weight array,
+        // which holds the rows' weight for one conw_r at a time (the J-free kernels form it themselves)
+        bool changed = false;
+        for (int i = 0; i < nh; ++i) {
+            changed |= b->hs[i]->conw_loaded != conw_r[i];
+            rc = set_conw(b->hs[i], conw_r[i]);
+            if (rc != PS_OK) return rc;
+        }
+        if (changed)
+            for (int i = 0; i < nh; ++i) HIPCHECK(hipStreamSynchronize(b->hs[i]->stream));
+    }
     // the job queue: problems without the Lambert column first, then those with it; each group longest first
     std::vector<int> order(njobs);
     for (int j = 0; j < njobs; ++j) order[j] = j;
@@ -1093,13 +1143,23 @@ extern "C" int ps_batch_run(ps_batch *b, const double *conw_r, int njobs, const 
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(psk::queue_reset, dim3(1), dim3(64), 0, b->stream, b->d_counter, 2);
-        if (n0)
-            hipLaunchKernelGGL(psk::queue_kernel2<false>, dim3(std::min(n0, b->slots)), dim3(psk::kWG2), 0, b->stream,
-                               b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs, n0, b->d_counter, b->d_out);
-        if (n1)
-            hipLaunchKernelGGL(psk::queue_kernel2<true>, dim3(std::min(n1, b->slots)), dim3(psk::kWG2), 0, b->stream,
-                               b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs + n0, n1, b->d_counter + 1,
-                               b->d_out);
+        if (b->mixed) {
+            if (n0)
+                hipLaunchKernelGGL(psk::queue_kernel_mixed<false>, dim3(std::min(n0, b->slots)), dim3(kWG), 0, b->stream,
+                                   b->d_probs, b->d_layout, b->d_conw, b->lpool, b->d_jobs, n0, b->d_counter, b->d_out);
+            if (n1)
+                hipLaunchKernelGGL(psk::queue_kernel_mixed<true>, dim3(std::min(n1, b->slots)), dim3(kWG), 0, b->stream,
+                                   b->d_probs, b->d_layout, b->d_conw, b->lpool, b->d_jobs + n0, n1, b->d_counter + 1,
+                                   b->d_out);
+        } else {
+            if (n0)
+                hipLaunchKernelGGL(psk::queue_kernel2<false>, dim3(std::min(n0, b->slots)), dim3(psk::kWG2), 0, b->stream,
+                                   b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs, n0, b->d_counter, b->d_out);
+            if (n1)
+                hipLaunchKernelGGL(psk::queue_kernel2<true>, dim3(std::min(n1, b->slots)), dim3(psk::kWG2), 0, b->stream,
+                                   b->d_probs, b->d_layout, b->d_conw, b->pool, b->d_jobs + n0, n1, b->d_counter + 1,
+                                   b->d_out);
+        }
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
         if (g) hipGraphDestroy(g);

@@ -990,6 +990,42 @@ __global__ __launch_bounds__(kWG2, 2) void queue_kernel2(const DevProblem *const
     }
 }
 
+// The same persistent queue in mixed precision: the materialised-Jacobian fit (FP32 facet loop on
+// FP32 MFMA with the FP32 Dsph table in LDS, FP64 normal equations and solve, ps_kernels.h), one
+// 512-lane workgroup per CU, each slot's Jacobian buffer sized for the largest problem.  Coefficient
+// layouts 0-3 (Lmax <= 6).
+template <bool LS>
+__global__ __launch_bounds__(kWG) void queue_kernel_mixed(const DevProblem *const *probs, const int *layout,
+                                                          const double *conw, Slots S, const QueueJob *jobs, int njobs,
+                                                          int *counter, ps_fit_result *out) {
+    __shared__ Shared sh;
+    const SlotPtrsL sp = slot_ptrs_legacy(S, blockIdx.x);
+    int prev = -1;
+    for (;;) {
+        if (threadIdx.x == 0) sh.job = atomicAdd(counter, 1);
+        __syncthreads();
+        const int j = __builtin_amdgcn_readfirstlane(sh.job);
+        __syncthreads();
+        if (j >= njobs) break;
+        const int prob = __builtin_amdgcn_readfirstlane(jobs[j].prob);
+        const int pole = __builtin_amdgcn_readfirstlane(jobs[j].pole);
+        const int oi = __builtin_amdgcn_readfirstlane(jobs[j].out);
+        const double freq = jobs[j].freq;
+        const DevProblem &P = *uniform_ptr(probs[prob]);
+        if (prob != prev) {  // the next fit's first barrier publishes the normals
+            load_facets(P, sh);
+            prev = prob;
+        }
+        const double cw = conw[prob];
+        switch (__builtin_amdgcn_readfirstlane(layout[prob])) {
+            case 0: fit_one_legacy<1, 0, true, true, LS>(P, sh, sp, freq, pole, cw, out + oi); break;
+            case 1: fit_one_legacy<1, 9, true, true, LS>(P, sh, sp, freq, pole, cw, out + oi); break;
+            case 2: fit_one_legacy<2, 4, true, true, LS>(P, sh, sp, freq, pole, cw, out + oi); break;
+            default: fit_one_legacy<3, 1, true, true, LS>(P, sh, sp, freq, pole, cw, out + oi); break;
+        }
+    }
+}
+
 // Resets the queue counters of a launch (a kernel rather than a memset: the launch sequence is
 // captured in a graph).
 __global__ void queue_reset(int *counter, int n) {

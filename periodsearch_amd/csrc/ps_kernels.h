@@ -101,6 +101,7 @@ struct Shared {
     int ipiv[64];
     double scal[8];
     int iscal[4];
+    int job;                // the queue kernel: the job claimed by the workgroup
     int next_tp;            // phase B: the next unclaimed tile pair
     double bl[9], dbl[18];  // pole rotation and its beta/lambda derivatives (blmatrix)
     double hbl[18];         // dbl_j * bl^T: the beta/lambda derivative generators in the pole frame
@@ -1089,8 +1090,185 @@ __device__ inline void load_facets(const DevProblem &P, Shared &sh) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// One (period, pole) fit per workgroup iteration: period_search_BOINC.cpp:819-911 with mrqmin.c.
+// One (period, pole) fit: period_search_BOINC.cpp:819-911 with mrqmin.c.  The slot's buffers: the
+// Jacobian rows, the model values, the per-segment column sums, and alpha/beta double-buffered
+// (accepted / trial).  sh.nx/ny/nz hold the problem's normals.
 // ---------------------------------------------------------------------------------------------
+struct SlotPtrsL {
+    double *J, *ytemp, *dave, *alpha0, *alpha1, *beta0, *beta1;
+};
+__device__ inline SlotPtrsL slot_ptrs_legacy(const Slots &S, int slot) {
+    SlotPtrsL p;
+    p.J = S.J + S.j_stride * slot;
+    p.ytemp = S.ytemp + S.y_stride * slot;
+    p.dave = S.dave + S.dave_stride * slot;
+    p.alpha0 = S.alpha + S.a_stride * (2 * size_t(slot));
+    p.alpha1 = p.alpha0 + S.a_stride;
+    p.beta0 = S.beta + S.b_stride * (2 * size_t(slot));
+    p.beta1 = p.beta0 + S.b_stride;
+    return p;
+}
+
+template <int NT, int R, bool LDSB, bool MIX, bool LS>
+__device__ __forceinline__ void fit_one_legacy(const DevProblem &P, Shared &sh, const SlotPtrsL &sp, double freq,
+                                               int pole, double conw_r, ps_fit_result *out) {
+    const int tid = threadIdx.x;
+    const int nc = P.nc, ma = P.ma, mfit = P.mfit;
+    double *const J = sp.J, *const ytemp = sp.ytemp, *const dave = sp.dave;
+    double *const alpha0 = sp.alpha0, *const alpha1 = sp.alpha1, *const beta0 = sp.beta0, *const beta1 = sp.beta1;
+    // initial parameters (:819-844)
+    if (tid < ma) {
+        double v;
+        const int m = pole - 1;
+        if (tid < nc)
+            v = P.cg_first[tid];
+        else if (tid == nc)
+            v = P.pole_beta[m];
+        else if (tid == nc + 1)
+            v = P.pole_lambda[m];
+        else if (tid == nc + 2)
+            v = 24 * 2 * kPi / (1 / freq);
+        else if (tid < nc + 6)
+            v = (tid == nc + 3) ? P.par1 : (tid == nc + 4) ? P.par2 : P.par3;
+        else if (tid == nc + 6)
+            v = P.log_cl;
+        else
+            v = 1;
+        sh.cg[tid] = v;
+    }
+    __syncthreads();
+    int cur = 0;
+    double alamda = -1, ochisq = 0, chisq = 0, rchisq = -1;
+    double iter_diff = 1e40, dev_old = 1e30, dev_new = 0;
+    double chck0 = 0, chck1 = 0, chck2 = 0;
+    int niter = 0, nfail = 0;
+    // the LM state is wave-uniform; its values meet at the joins of lane-predicated stores
+    // (tid < ma), so settle() re-asserts it after each such region: the loop exit and the
+    // accept / reject decisions then branch on scalars
+    auto settle = [&] {
+        alamda = uniform_double(alamda);
+        ochisq = uniform_double(ochisq);
+        chisq = uniform_double(chisq);
+        rchisq = uniform_double(rchisq);
+        iter_diff = uniform_double(iter_diff);
+        dev_old = uniform_double(dev_old);
+        dev_new = uniform_double(dev_new);
+        chck0 = uniform_double(chck0);
+        chck1 = uniform_double(chck1);
+        chck2 = uniform_double(chck2);
+        niter = __builtin_amdgcn_readfirstlane(niter);
+        nfail = __builtin_amdgcn_readfirstlane(nfail);
+        cur = __builtin_amdgcn_readfirstlane(cur);
+    };
+    for (;;) {
+        settle();
+        if (!(((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min))) break;
+        // One mrqcof per pass (a single inlined copy): the first pass of a fit evaluates the
+        // starting point (mrqmin.c:34-52) and goes straight on to the step; later passes solve the
+        // damped system and evaluate the trial point (mrqmin.c:53-79).
+        const bool init = alamda < 0;
+        int err = 0;
+        if (init) {
+            alamda = P.alamda_start;
+            if (tid < ma) sh.a[tid] = sh.cg[tid];
+        } else {
+            for (int it = tid; it < mfit * mfit; it += kWG) {
+                const int j = it / mfit, k = it - j * mfit;
+                const double v = (cur ? alpha1 : alpha0)[it];
+                const double d = (j == k) ? v * (1 + alamda) : v;
+                sh.u.g.cov[it] = d;
+                if (MIX) sh.u.g.cov0[it] = d;
+            }
+            for (int j = tid; j < mfit; j += kWG) {
+                sh.da[j] = (cur ? beta1 : beta0)[j];
+                if (MIX) sh.b0[j] = sh.da[j];
+            }
+            __syncthreads();
+            err = __builtin_amdgcn_readfirstlane(gauss_jordan(sh, mfit));
+            __syncthreads();
+            if (MIX && err == 0) refine_solution(sh, mfit);
+            if (err == 0) {
+                if (tid < mfit) {
+                    const int l = P.fit_idx[tid];
+                    sh.atry[l] = sh.cg[l] + sh.da[tid];
+                }
+                __syncthreads();
+                if (tid < ma) sh.a[tid] = sh.atry[tid];
+            }
+        }
+        if (err == 0) {
+            __syncthreads();
+            // the starting point fills the current buffers, a trial point the other ones
+            const bool other = !init;
+            const double chi = uniform_double(mrqcof<NT, R, LDSB, MIX, LS>(
+                P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0, (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr));
+            if (init) {
+                ochisq = chi;
+                chisq = chi;  // a failed first step then reports this fit's own starting chi2
+                if (tid < ma) sh.atry[tid] = sh.cg[tid];
+            } else {
+                chisq = chi;
+                if (chisq < ochisq) {
+                    alamda = alamda / P.alamda_incr;
+                    cur ^= 1;
+                    if (tid < ma) sh.cg[tid] = sh.atry[tid];
+                } else {
+                    alamda = P.alamda_incr * alamda;
+                    chisq = ochisq;
+                }
+            }
+            __syncthreads();
+        } else {
+            ++nfail;
+        }
+        settle();
+        if (init) continue;  // the starting point is evaluated; the first step follows
+        ++niter;
+        if (niter == 1 || chisq < ochisq) {
+            ochisq = chisq;
+            if (tid < ma) sh.a[tid] = sh.cg[tid];
+            __syncthreads();
+            curv_lds(P, sh.a, sh);
+            __syncthreads();
+            if (tid < 3) {
+                const double *nn = (tid == 0) ? sh.nx : (tid == 1) ? sh.ny : sh.nz;
+                double s = 0;
+                for (int j = 0; j < P.nf; ++j) s = s + sh.area[j] * nn[j];
+                sh.scal[1 + tid] = s;
+            }
+            __syncthreads();
+            chck0 = uniform_double(sh.scal[1]);
+            chck1 = uniform_double(sh.scal[2]);
+            chck2 = uniform_double(sh.scal[3]);
+            rchisq = chisq - (chck0 * chck0 + chck1 * chck1 + chck2 * chck2) * (conw_r * conw_r);
+            __syncthreads();
+        }
+        settle();
+        dev_new = sqrt(rchisq / P.nobs);
+        if (dev_old - dev_new > 1e-10) {
+            iter_diff = dev_old - dev_new;
+            dev_old = dev_new;
+        }
+    }
+    if (tid == 0) {
+        double totarea = 0;
+        for (int j = 0; j < P.nf; ++j) totarea = totarea + sh.area[j];
+        ps_fit_result r;
+        r.dev = dev_new;
+        r.prd = 2 * kPi / sh.cg[nc + 2];
+        r.dark = sqrt(chck0 * chck0 + chck1 * chck1 + chck2 * chck2);
+        r.totarea = totarea;
+        r.la = (180 / kPi) * sh.cg[nc + 1];
+        r.be = 90 - (180 / kPi) * sh.cg[nc];
+        r.chisq = chisq;
+        r.niter = niter;
+        r.nsolve_fail = nfail;
+        *out = r;
+    }
+    __syncthreads();
+}
+
+// The fits of one problem, a grid-stride loop over them.
 template <int NT, int R, bool LDSB, bool MIX, bool LS>
 __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__ Pp, Slots S, const double *freq,
                                                   const int *pole, int nfits, double conw_r, ps_fit_result *out) {
@@ -1098,167 +1276,10 @@ __global__ __launch_bounds__(kWG) void fit_kernel(const DevProblem *__restrict__
     // by value: ~40 kernel-argument pointers held in SGPRs would spill into VGPRs.
     const DevProblem &P = *Pp;
     __shared__ Shared sh;
-    const int tid = threadIdx.x;
-    const int nc = P.nc, ma = P.ma, mfit = P.mfit;
-    double *J = S.J + S.j_stride * blockIdx.x;
-    double *ytemp = S.ytemp + S.y_stride * blockIdx.x;
-    double *dave = S.dave + S.dave_stride * blockIdx.x;
-    double *const alpha0 = S.alpha + S.a_stride * (2 * blockIdx.x), *const alpha1 = alpha0 + S.a_stride;
-    double *const beta0 = S.beta + S.b_stride * (2 * blockIdx.x), *const beta1 = beta0 + S.b_stride;
+    const SlotPtrsL sp = slot_ptrs_legacy(S, blockIdx.x);
     load_facets(P, sh);
-
-    for (int fi = blockIdx.x; fi < nfits; fi += gridDim.x) {
-        // initial parameters (:819-844)
-        if (tid < ma) {
-            double v;
-            const int m = pole[fi] - 1;
-            if (tid < nc)
-                v = P.cg_first[tid];
-            else if (tid == nc)
-                v = P.pole_beta[m];
-            else if (tid == nc + 1)
-                v = P.pole_lambda[m];
-            else if (tid == nc + 2)
-                v = 24 * 2 * kPi / (1 / freq[fi]);
-            else if (tid < nc + 6)
-                v = (tid == nc + 3) ? P.par1 : (tid == nc + 4) ? P.par2 : P.par3;
-            else if (tid == nc + 6)
-                v = P.log_cl;
-            else
-                v = 1;
-            sh.cg[tid] = v;
-        }
-        __syncthreads();
-        int cur = 0;
-        double alamda = -1, ochisq = 0, chisq = 0, rchisq = -1;
-        double iter_diff = 1e40, dev_old = 1e30, dev_new = 0;
-        double chck0 = 0, chck1 = 0, chck2 = 0;
-        int niter = 0, nfail = 0;
-        // the LM state is wave-uniform; its values meet at the joins of lane-predicated stores
-        // (tid < ma), so settle() re-asserts it after each such region: the loop exit and the
-        // accept / reject decisions then branch on scalars
-        auto settle = [&] {
-            alamda = uniform_double(alamda);
-            ochisq = uniform_double(ochisq);
-            chisq = uniform_double(chisq);
-            rchisq = uniform_double(rchisq);
-            iter_diff = uniform_double(iter_diff);
-            dev_old = uniform_double(dev_old);
-            dev_new = uniform_double(dev_new);
-            chck0 = uniform_double(chck0);
-            chck1 = uniform_double(chck1);
-            chck2 = uniform_double(chck2);
-            niter = __builtin_amdgcn_readfirstlane(niter);
-            nfail = __builtin_amdgcn_readfirstlane(nfail);
-            cur = __builtin_amdgcn_readfirstlane(cur);
-        };
-        for (;;) {
-            settle();
-            if (!(((niter < P.n_iter_max) && (iter_diff > P.iter_diff_max)) || (niter < P.n_iter_min))) break;
-            // One mrqcof per pass (a single inlined copy): the first pass of a fit evaluates the
-            // starting point (mrqmin.c:34-52) and goes straight on to the step; later passes solve the
-            // damped system and evaluate the trial point (mrqmin.c:53-79).
-            const bool init = alamda < 0;
-            int err = 0;
-            if (init) {
-                alamda = P.alamda_start;
-                if (tid < ma) sh.a[tid] = sh.cg[tid];
-            } else {
-                for (int it = tid; it < mfit * mfit; it += kWG) {
-                    const int j = it / mfit, k = it - j * mfit;
-                    const double v = (cur ? alpha1 : alpha0)[it];
-                    const double d = (j == k) ? v * (1 + alamda) : v;
-                    sh.u.g.cov[it] = d;
-                    if (MIX) sh.u.g.cov0[it] = d;
-                }
-                for (int j = tid; j < mfit; j += kWG) {
-                    sh.da[j] = (cur ? beta1 : beta0)[j];
-                    if (MIX) sh.b0[j] = sh.da[j];
-                }
-                __syncthreads();
-                err = __builtin_amdgcn_readfirstlane(gauss_jordan(sh, mfit));
-                __syncthreads();
-                if (MIX && err == 0) refine_solution(sh, mfit);
-                if (err == 0) {
-                    if (tid < mfit) {
-                        const int l = P.fit_idx[tid];
-                        sh.atry[l] = sh.cg[l] + sh.da[tid];
-                    }
-                    __syncthreads();
-                    if (tid < ma) sh.a[tid] = sh.atry[tid];
-                }
-            }
-            if (err == 0) {
-                __syncthreads();
-                // the starting point fills the current buffers, a trial point the other ones
-                const bool other = !init;
-                const double chi = uniform_double(mrqcof<NT, R, LDSB, MIX, LS>(
-                    P, J, ytemp, dave, (cur ^ other) ? alpha1 : alpha0, (cur ^ other) ? beta1 : beta0, sh, nullptr, nullptr));
-                if (init) {
-                    ochisq = chi;
-                    chisq = chi;  // a failed first step then reports this fit's own starting chi2
-                    if (tid < ma) sh.atry[tid] = sh.cg[tid];
-                } else {
-                    chisq = chi;
-                    if (chisq < ochisq) {
-                        alamda = alamda / P.alamda_incr;
-                        cur ^= 1;
-                        if (tid < ma) sh.cg[tid] = sh.atry[tid];
-                    } else {
-                        alamda = P.alamda_incr * alamda;
-                        chisq = ochisq;
-                    }
-                }
-                __syncthreads();
-            } else {
-                ++nfail;
-            }
-            settle();
-            if (init) continue;  // the starting point is evaluated; the first step follows
-            ++niter;
-            if (niter == 1 || chisq < ochisq) {
-                ochisq = chisq;
-                if (tid < ma) sh.a[tid] = sh.cg[tid];
-                __syncthreads();
-                curv_lds(P, sh.a, sh);
-                __syncthreads();
-                if (tid < 3) {
-                    const double *nn = (tid == 0) ? sh.nx : (tid == 1) ? sh.ny : sh.nz;
-                    double s = 0;
-                    for (int j = 0; j < P.nf; ++j) s = s + sh.area[j] * nn[j];
-                    sh.scal[1 + tid] = s;
-                }
-                __syncthreads();
-                chck0 = uniform_double(sh.scal[1]);
-                chck1 = uniform_double(sh.scal[2]);
-                chck2 = uniform_double(sh.scal[3]);
-                rchisq = chisq - (chck0 * chck0 + chck1 * chck1 + chck2 * chck2) * (conw_r * conw_r);
-                __syncthreads();
-            }
-            settle();
-            dev_new = sqrt(rchisq / P.nobs);
-            if (dev_old - dev_new > 1e-10) {
-                iter_diff = dev_old - dev_new;
-                dev_old = dev_new;
-            }
-        }
-        if (tid == 0) {
-            double totarea = 0;
-            for (int j = 0; j < P.nf; ++j) totarea = totarea + sh.area[j];
-            ps_fit_result r;
-            r.dev = dev_new;
-            r.prd = 2 * kPi / sh.cg[nc + 2];
-            r.dark = sqrt(chck0 * chck0 + chck1 * chck1 + chck2 * chck2);
-            r.totarea = totarea;
-            r.la = (180 / kPi) * sh.cg[nc + 1];
-            r.be = 90 - (180 / kPi) * sh.cg[nc];
-            r.chisq = chisq;
-            r.niter = niter;
-            r.nsolve_fail = nfail;
-            out[fi] = r;
-        }
-        __syncthreads();
-    }
+    for (int fi = blockIdx.x; fi < nfits; fi += gridDim.x)
+        fit_one_legacy<NT, R, LDSB, MIX, LS>(P, sh, sp, freq[fi], pole[fi], conw_r, out + fi);
 }
 
 // mrqcof for externally supplied parameter vectors (ps_eval).
