@@ -179,8 +179,9 @@ int ps_fit_batch_noalloc(ps_handle *h, int nfits, const double *freq, const int 
 /* A batch of many problems (asteroids) on one device, for repeated runs.  The reference runs one
    asteroid per process and has no counterpart; the per-job body is period_search_BOINC.cpp:817-921.
    ps_batch_create: the problem table on the device and one pool of per-workgroup buffers sized for the
-   largest problem (the problems must run the FP64 J-free kernels: precision PS_FP64 and no relative
-   lightcurve longer than 128 points; else PS_ERR_UNSUPPORTED).  The handles must outlive the batch.
+   largest problem.  All handles at PS_FP64: the J-free kernels (no relative lightcurve longer than 128
+   points); all at PS_MIXED: the mixed-precision kernels (Lmax <= 6); otherwise PS_ERR_UNSUPPORTED or
+   PS_ERR_ARG (precisions differ).  The handles must outlive the batch.
    ps_batch_run: njobs fits (job j: problem prob[j], frequency freq[j] in 1/h, pole pole[j] in 1..10, at
    convexity weight conw_r[prob[j]]) as one graph launch: a queue reset, then persistent workgroups
    that claim jobs from a device work queue, longest first, until it is empty.  out[j] equals
