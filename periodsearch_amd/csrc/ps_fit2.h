@@ -391,7 +391,8 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
     const int pl = c & 15, kg = c >> 4;
     const int mfit = P.mfit, c00col = P.c00col;
     const int *unit_rows = P.unit_rows;
-    const bool incol = c <= mfit, zero_col = c == c00col, res_col = c == mfit;
+    // the columns the row loop writes: the residual and c00 columns are written separately
+    const bool wcol = c < mfit && c != c00col;
     const int cl = min(c, mfit);  // the column a lane reads (lanes past mfit read a valid one, unused)
     // the summed column, clamped: lanes past mfit add a valid column and their sums are not used
     const int scol = min(16 * wave + pl, mfit);
@@ -416,20 +417,30 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
         const double rave = 1 / ave;
         const double coef_a = rd.sl_a / ave, coef_b = rd.sl_b / ave;
         const double res_a = rd.y_a - coef_a * ym_a, res_b = rd.y_b - coef_b * ym_b;
-        // row r's factors from lane r (rows below 64) or r - 64: the two halves as two loops, so the
-        // lane reads need no selects
-        auto update = [&](int r, double coef, double ym, double res) {
+        // The residual column and the c00 column (0) of row r: by lane r (or r - 64) of the wave that
+        // normalises the row, so that every row is written by one wave.
+        auto fixed_cols = [&](int r, double res) {
+            if (r >= r0 && r < r1 && (r - r0) % kWaves2 == wave) {
+                double *row = sh.u.rows + r * kJP;
+                row[mfit] = res;
+                if (c00col >= 0) row[c00col] = 0.0;
+            }
+        };
+        fixed_cols(c, res_a);
+        fixed_cols(64 + c, res_b);
+        // the other columns of row r, with the row's factors from lane r (rows below 64) or r - 64: the
+        // two halves as two loops, so the lane reads need no selects
+        auto update = [&](int r, double coef, double ym) {
             double *row = sh.u.rows + r * kJP;
-            const double x = row[cl];
-            const double v = res_col ? res : zero_col ? 0.0 : coef * (x - (ym * s) * rave);
-            if (incol) row[c] = v;
+            const double v = coef * (row[cl] - (ym * s) * rave);
+            if (wcol) row[c] = v;
         };
         int r = r0 + wave;
         const int rm = min(r1, 64);
 #pragma unroll 2
-        for (; r < rm; r += kWaves2) update(r, read_lane(coef_a, r), read_lane(ym_a, r), read_lane(res_a, r));
+        for (; r < rm; r += kWaves2) update(r, read_lane(coef_a, r), read_lane(ym_a, r));
 #pragma unroll 2
-        for (; r < r1; r += kWaves2) update(r, read_lane(coef_b, r - 64), read_lane(ym_b, r - 64), read_lane(res_b, r - 64));
+        for (; r < r1; r += kWaves2) update(r, read_lane(coef_b, r - 64), read_lane(ym_b, r - 64));
     }
 }
 
@@ -687,11 +698,12 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
         const int irow = cq / n, icol = cq - irow * n;
         done |= 1ull << icol;
         const double *prow = sh.u.g.cand[buf][ww];
-        const double s = uniform_double(prow[icol]);
+        const double pv = prow[l];  // one read of the pivot row: the pivot is its lane icol
+        const double s = read_lane(pv, icol);
         if (s == 0.0) return 2;
         const double pivinv = 1.0 / s;
         // the new pivot row, gauss_errc.c:44-53 (lanes past n are not read)
-        const double pr = ((l == icol) ? 1.0 : prow[l]) * pivinv;
+        const double pr = ((l == icol) ? 1.0 : pv) * pivinv;
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
         const bool piv_lane = l == icol;
@@ -710,7 +722,7 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
             const double dum = read_lane(cur, icol);
             const double v = (piv_lane ? 0.0 : cur) - pr * dum;
             row[k] = v;
-            if (!((done >> lab[k]) & 1) && cand_col) pivot_merge(c, {fabs(v), lab[k] * n + l});
+            if (!((done >> lab[k]) & 1)) pivot_merge(c, {fabs(v), lab[k] * n + l}, cand_col);  // (uniform test)
         }
     }
 #pragma unroll

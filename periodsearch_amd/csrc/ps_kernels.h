@@ -878,13 +878,19 @@ struct PivotCand {
 
 // A lane visits its candidates in increasing index order (rows ascending, its column fixed), so
 // "larger |a|, ties to the larger index" is |a| >= the best so far.
+// (Selects, not branches: the choice is per lane, and a branch would switch the execution mask.)
 __device__ inline void pivot_take(PivotCand &c, double s, int q) {
     const double v = fabs(s);
-    if (v >= c.v) c = {v, q};
+    const bool t = v >= c.v;
+    c.v = t ? v : c.v;
+    c.q = t ? q : c.q;
 }
 
-__device__ inline void pivot_merge(PivotCand &c, const PivotCand &o) {
-    if (o.v > c.v || (o.v == c.v && o.q > c.q)) c = o;
+// c = the larger of c and o, if e
+__device__ inline void pivot_merge(PivotCand &c, const PivotCand &o, bool e = true) {
+    const bool t = e & ((o.v > c.v) | ((o.v == c.v) & (o.q > c.q)));
+    c.v = t ? o.v : c.v;
+    c.q = t ? o.q : c.q;
 }
 
 template <int CTRL>
@@ -1009,11 +1015,12 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
         }
         done |= 1ull << icol;
         const double *prow = sh.u.g.cand[buf][ww];
-        const double s = uniform_double(prow[icol]);
+        const double pv = prow[l];  // one read of the pivot row: the pivot is its lane icol
+        const double s = read_lane(pv, icol);
         if (s == 0.0) return 2;  // the same in every wave
         const double pivinv = 1.0 / s;
         // the new pivot row (gauss_errc.c:44-53; lanes past n are not read)
-        const double pr = ((l == icol) ? 1.0 : prow[l]) * pivinv;
+        const double pr = ((l == icol) ? 1.0 : pv) * pivinv;
         // eliminate column icol from every other row (gauss_errc.c:55-62), searching the next pivot
         c = {-1.0, -1};
         const bool cand_col = l < n && !((done >> l) & 1);
@@ -1032,7 +1039,7 @@ __device__ inline int gauss_jordan(Shared &sh, int n) {
             const double dum = read_lane(cur, icol);        // the pivot-column entry of the row
             const double v = (piv_lane ? 0.0 : cur) - pr * dum;
             row[k] = v;
-            if (!((done >> lab[k]) & 1) && cand_col) pivot_merge(c, {fabs(v), lab[k] * n + l});
+            if (!((done >> lab[k]) & 1)) pivot_merge(c, {fabs(v), lab[k] * n + l}, cand_col);  // (uniform test)
         }
     }
 #pragma unroll
