@@ -166,7 +166,6 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
 #pragma unroll
             for (int r = 0; r < R; ++r) ts[h].racc[r] = 0;
             ts[h].d1 = ts[h].d2 = 0;
-            ts[h].br = 0;
         }
         // Dsph through L2: the operands of chunk c + 1 are loaded while chunk c is computed (the table
         // has a zero chunk of facets past its end, so the last chunk's load needs no branch)
@@ -207,7 +206,6 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
                 const double muv = vis * mu, mu0v = vis * mu0;
                 const double mm = muv * mu0;
                 const double X = mm * fma(As, rd, Ac);
-                s.br += X;
                 if constexpr (LS) {
                     s.d1 += A * mm;
                     s.d2 += A * (mm * rd);
@@ -250,7 +248,13 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
             TileSums<NT, R, false> &s = ts[h];
             const int tile = tl + h;
             const int q = (tile0 + tile) * 16 + pl;
-            const double br = sum_groups_lo(s.br);
+            // the brightness: the MFMA's coefficient-0 column (Dsph_0 = 1 on every facet), point 4r + g
+            // in register r of lane 16g
+            double b4[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) b4[r] = __shfl(s.acc[0][r], 16 * (pl & 3));
+            const int rb = pl >> 2;
+            const double br = rb == 0 ? b4[0] : rb == 1 ? b4[1] : rb == 2 ? b4[2] : b4[3];
             double S[3], U[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
