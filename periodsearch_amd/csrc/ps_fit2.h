@@ -71,21 +71,7 @@ template <int NFM>
 __device__ inline void curv2(const DevProblem &P, const double *a, Shared2<NFM> &sh, double *area) {
     const double cl = exp(a[P.nc + 6]), cls = a[P.nc + 7];
     for (int i = threadIdx.x; i < P.nf; i += kWG2) {
-        double gsum = 0;
-        int n = 0;
-        const double *pl = P.pleg + size_t(i) * P.npairs;
-        const double *fc = P.fc + size_t(i) * P.mmax1, *fs = P.fs + size_t(i) * P.mmax1;
-        for (int q = 0; q < P.npairs; ++q) {
-            const int m = gload(P.pair_m, q);
-            double fsum = a[n] * gload(fc, m);
-            ++n;
-            if (m != 0) {
-                fsum = fsum + a[n] * gload(fs, m);
-                ++n;
-            }
-            gsum = gsum + gload(pl, q) * fsum;
-        }
-        const double ar = gload(P.darea, i) * exp(gsum);
+        const double ar = gload(P.darea, i) * exp(facet_shape_sum(P, a, i));
         area[i] = ar;
         sh.acl[i] = ar * cl;
         sh.acls[i] = ar * cls;

@@ -162,24 +162,52 @@ __device__ inline T gload(const T *p, size_t i) {
 // curv (period_search/curv.c:10-39): g = exp(sum_{m,l} Pleg*(c*Fc + s*Fs)), Area = Darea*g, and the
 // facet loop's Area*cl and Area*cls.
 // ---------------------------------------------------------------------------------------------
+// A wave-uniform read of read-only problem data through the scalar cache (s_load): unlike a vector
+// load it does not wait behind the wave's outstanding vector stores.
+__device__ inline int uload(const int *p, int i) {
+    return reinterpret_cast<const __attribute__((address_space(4))) int *>(reinterpret_cast<unsigned long long>(p))[i];
+}
+
+// The exponent of facet i's area factor, sum over the (l, m) pairs of Pleg (a_n Fc_m + a_n+1 Fs_m) in
+// the order of curv.c:19-31.  The facet's table entries are loaded four pairs ahead of the sums (the
+// loads do not depend on them), so their latencies overlap instead of adding up pair by pair.
+__device__ inline double facet_shape_sum(const DevProblem &P, const double *a, int i) {
+    const int np = P.npairs;
+    const double *pl = P.pleg + size_t(i) * np;
+    const double *fc = P.fc + size_t(i) * P.mmax1, *fs = P.fs + size_t(i) * P.mmax1;
+    double gsum = 0;
+    int n = 0;
+    for (int q0 = 0; q0 < np; q0 += 4) {
+        double vp[4], vc[4], vs[4];
+        int mm[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = min(q0 + j, np - 1);  // clamped: the loads past the last pair are not used
+            mm[j] = uload(P.pair_m, q);
+            vp[j] = gload(pl, q);
+            vc[j] = gload(fc, mm[j]);
+            vs[j] = gload(fs, mm[j]);  // m = 0 reads Fs_0, unused
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (q0 + j < np) {
+                double fsum = a[n] * vc[j];
+                ++n;
+                if (mm[j] != 0) {
+                    fsum = fsum + a[n] * vs[j];
+                    ++n;
+                }
+                gsum = gsum + vp[j] * fsum;
+            }
+        }
+    }
+    return gsum;
+}
+
 __device__ inline void curv_lds(const DevProblem &P, const double *a, Shared &sh) {
     const double cl = exp(a[P.nc + 6]), cls = a[P.nc + 7];
     for (int i = threadIdx.x; i < P.nf; i += kWG) {
-        double gsum = 0;
-        int n = 0;
-        const double *pl = P.pleg + size_t(i) * P.npairs;
-        const double *fc = P.fc + size_t(i) * P.mmax1, *fs = P.fs + size_t(i) * P.mmax1;
-        for (int q = 0; q < P.npairs; ++q) {
-            const int m = gload(P.pair_m, q);
-            double fsum = a[n] * gload(fc, m);
-            ++n;
-            if (m != 0) {
-                fsum = fsum + a[n] * gload(fs, m);
-                ++n;
-            }
-            gsum = gsum + gload(pl, q) * fsum;
-        }
-        const double ar = gload(P.darea, i) * exp(gsum);
+        const double ar = gload(P.darea, i) * exp(facet_shape_sum(P, a, i));
         sh.area[i] = ar;
         sh.acl[i] = ar * cl;
         sh.acls[i] = ar * cls;
@@ -297,12 +325,6 @@ __device__ inline void spin_columns(const SH &sh, const double e[3], const doubl
         js[j] = (S[0] * d[0] + S[1] * d[1] + S[2] * d[2]) + (U[0] * d0[0] + U[1] * d0[1] + U[2] * d0[2]);
     }
     js[2] = t * ((S[0] * e[1] - S[1] * e[0]) + (U[0] * e0[1] - U[1] * e0[0]));
-}
-
-// A wave-uniform read of read-only problem data through the scalar cache (s_load): unlike a vector
-// load it does not wait behind the wave's outstanding vector stores.
-__device__ inline int uload(const int *p, int i) {
-    return reinterpret_cast<const __attribute__((address_space(4))) int *>(reinterpret_cast<unsigned long long>(p))[i];
 }
 
 // Cross-lane sums on the VALU (no LDS round trip, no index registers).  v_permlane16/32_swap give each
