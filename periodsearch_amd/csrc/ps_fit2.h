@@ -489,10 +489,15 @@ __device__ __forceinline__ void gram_pass_w(const DevProblem &P, const Shared2<N
     const double *base = sh.u.rows + kg * kJP;
     for (int t = 0; t < ntl; ++t) {  // one tile: four K-steps of four rows
         const double wt = t < 4 ? rd.w_a : rd.w_b;
+        // the weights of the lane's rows in the four K-steps, fetched together (one LDS-crossbar
+        // latency per tile instead of one per K-step)
+        double w4[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) w4[s4] = __shfl(wt, (16 * t + 4 * s4 + kg) & 63);
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
             const int r = 16 * t + 4 * s4;  // K-step rows r .. r + 3, the lane's row r + kg
-            const double w = __shfl(wt, (r + kg) & 63);
+            const double w = w4[s4];
             double v[4];
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) v[cb] = (used & (1 << cb)) ? base[r * kJP + col[cb]] : 0.0;
