@@ -392,8 +392,17 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
         // class kg: rows r0 + kg, r0 + kg + 4, ... of the whole groups of four, in order; the rows past
         // the last whole group go to class 0, in order
         const int nfull = (r1 - r0) >> 2;
+        const double *cp = sh.u.rows + (r0 + kg) * kJP + scol;  // the lane's class, row by row
         double sk = 0;
-        for (int i = 0; i < nfull; ++i) sk += sh.u.rows[(r0 + 4 * i + kg) * kJP + scol];
+        int i = 0;
+        for (; i + 4 <= nfull; i += 4) {  // four reads in flight, added in row order
+            double x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[j] = cp[(4 * (i + j)) * kJP];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sk += x[j];
+        }
+        for (; i < nfull; ++i) sk += cp[(4 * i) * kJP];
         for (int r = r0 + 4 * nfull; r < r1; ++r) {
             const double x = sh.u.rows[r * kJP + scol];
             sk += kg == 0 ? x : 0.0;
@@ -427,9 +436,9 @@ __device__ inline void normalise_pass(const DevProblem &P, Shared2<NFM> &sh, con
         };
         int r = r0 + wave;
         const int rm = min(r1, 64);
-#pragma unroll 2
+#pragma unroll 4
         for (; r < rm; r += kWaves2) update(r, read_lane(coef_a, r), read_lane(ym_a, r));
-#pragma unroll 2
+#pragma unroll 4
         for (; r < r1; r += kWaves2) update(r, read_lane(coef_b, r - 64), read_lane(ym_b, r - 64));
     }
 }
