@@ -182,29 +182,62 @@ __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &s
             for (int tt = 0; tt < NT; ++tt) Bn[tt] = dpl[16 * tt];
 #pragma unroll
             for (int r = 0; r < RP; ++r) Brn[r] = dpr[r];
+            // The tiles' dependency chains in lockstep, step by step (the same operations as one tile
+            // at a time): a wave alone on its SIMD has two independent instructions to issue at every
+            // step instead of waiting on each result.
+            double mu[KT], mu0[KT], vis[KT], dn[KT], rd[KT], er[KT];
+#pragma unroll
+            for (int h = 0; h < KT; ++h) {
+                mu[h] = e[h][0] * n0 + e[h][1] * n1 + e[h][2] * n2;
+                mu0[h] = e0[h][0] * n0 + e0[h][1] * n1 + e0[h][2] * n2;
+            }
+#pragma unroll
+            for (int h = 0; h < KT; ++h) {
+                vis[h] = (mu[h] > kTiny && mu0[h] > kTiny) ? 1.0 : 0.0;
+                dn[h] = fmax(mu[h] + mu0[h], kTiny);
+                rd[h] = __builtin_amdgcn_rcp(dn[h]);  // Prec<false>::rcp, step by step
+            }
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+#pragma unroll
+                for (int h = 0; h < KT; ++h) er[h] = fma(-dn[h], rd[h], 1.0);
+#pragma unroll
+                for (int h = 0; h < KT; ++h) rd[h] = fma(rd[h], er[h], rd[h]);
+            }
+            double muv[KT], mu0v[KT], mm[KT], X[KT], h2[KT], ad[KT], ad0[KT];
+#pragma unroll
+            for (int h = 0; h < KT; ++h) {
+                muv[h] = vis[h] * mu[h];
+                mu0v[h] = vis[h] * mu0[h];
+            }
+#pragma unroll
+            for (int h = 0; h < KT; ++h) {
+                mm[h] = muv[h] * mu0[h];
+                X[h] = mm[h] * fma(As, rd[h], Ac);
+                h2[h] = As * (rd[h] * rd[h]);
+            }
+#pragma unroll
+            for (int h = 0; h < KT; ++h) {
+                ad[h] = mu0v[h] * fma(h2[h], mu0[h], Ac);
+                ad0[h] = muv[h] * fma(h2[h], mu[h], Ac);
+            }
 #pragma unroll
             for (int h = 0; h < KT; ++h) {
                 TileSums<NT, R, false> &s = ts[h];
-                const double mu = e[h][0] * n0 + e[h][1] * n1 + e[h][2] * n2;
-                const double mu0 = e0[h][0] * n0 + e0[h][1] * n1 + e0[h][2] * n2;
-                const double vis = (mu > kTiny && mu0 > kTiny) ? 1.0 : 0.0;
-                const double rd = Pr::rcp(fmax(mu + mu0, kTiny));
-                const double muv = vis * mu, mu0v = vis * mu0;
-                const double mm = muv * mu0;
-                const double X = mm * fma(As, rd, Ac);
                 if constexpr (LS) {
-                    s.d1 += A * mm;
-                    s.d2 += A * (mm * rd);
+                    s.d1 += A * mm[h];
+                    s.d2 += A * (mm[h] * rd[h]);
                 }
-                const double h2 = As * (rd * rd);
-                const double ad = mu0v * fma(h2, mu0, Ac);
-                const double ad0 = muv * fma(h2, mu, Ac);
-                s.S[0] += ad * n0; s.S[1] += ad * n1; s.S[2] += ad * n2;
-                s.U[0] += ad0 * n0; s.U[1] += ad0 * n1; s.U[2] += ad0 * n2;
+                s.S[0] += ad[h] * n0; s.S[1] += ad[h] * n1; s.S[2] += ad[h] * n2;
+                s.U[0] += ad0[h] * n0; s.U[1] += ad0[h] * n1; s.U[2] += ad0[h] * n2;
+            }
 #pragma unroll
-                for (int tt = 0; tt < NT; ++tt) s.acc[tt] = Pr::mma(X, B[tt], s.acc[tt]);
+            for (int h = 0; h < KT; ++h) {
+                TileSums<NT, R, false> &s = ts[h];
 #pragma unroll
-                for (int r = 0; r < R; ++r) s.racc[r] += X * Br[r];
+                for (int tt = 0; tt < NT; ++tt) s.acc[tt] = Pr::mma(X[h], B[tt], s.acc[tt]);
+#pragma unroll
+                for (int r = 0; r < R; ++r) s.racc[r] += X[h] * Br[r];
             }
         }
         double scale[KT], dphp[KT][3];
