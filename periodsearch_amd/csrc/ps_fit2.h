@@ -100,6 +100,23 @@ __device__ inline double facet_sum(const double *area, const double *v, int nf) 
     return wave_sum(s);
 }
 
+// The three sums of the convexity check in one pass over the facets, each in facet_sum's order.
+__device__ inline void facet_sums_n(const double *area, const double *nx, const double *ny, const double *nz, int nf,
+                                    double s[3]) {
+    const int l = threadIdx.x & 63;
+    double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll 4
+    for (int i = l; i < nf; i += 64) {
+        const double ar = gload(area, i);
+        s0 += ar * nx[i];
+        s1 += ar * ny[i];
+        s2 += ar * nz[i];
+    }
+    s[0] = wave_sum(s0);
+    s[1] = wave_sum(s1);
+    s[2] = wave_sum(s2);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Phase B of one pass: the model rows of the wave's two tiles (pass-relative tiles 2w, 2w+1) into
 // sh.u.rows, and, in the last pass, the convexity rows.  The facet loop and epilogue arithmetic are
@@ -866,6 +883,7 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
         // passes solve the damped system and evaluate the trial point (mrqmin.c:53-79).
         const bool init = alamda < 0;
         int err = 0;
+        bool fresh = false;  // the step was accepted: cg is the trial point just evaluated
         if (init) {
             alamda = alamda_start;
             if (tid < ma) sh.a[tid] = sh.cg[tid];
@@ -904,6 +922,7 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
                 if (chisq < ochisq) {
                     alamda = alamda / alamda_incr;
                     cur ^= 1;
+                    fresh = true;
                     if (tid < ma) sh.cg[tid] = sh.atry[tid];
                 } else {
                     alamda = alamda_incr * alamda;
@@ -921,16 +940,17 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             ochisq = chisq;
             if (tid < ma) sh.a[tid] = sh.cg[tid];
             __syncthreads();
-            curv2(P, sh.a, sh, area);
+            // curv(cg) (period_search_BOINC.cpp:874); after an accepted step cg is the trial point, whose
+            // areas the evaluation has just computed from the same bits
+            if (!__builtin_amdgcn_readfirstlane(fresh)) curv2(P, sh.a, sh, area);
             __syncthreads();
             if (__builtin_amdgcn_readfirstlane(tid >> 6) == 0) {
-                const double s0 = facet_sum(area, sh.nx, P.nf);
-                const double s1 = facet_sum(area, sh.ny, P.nf);
-                const double s2 = facet_sum(area, sh.nz, P.nf);
+                double sn[3];
+                facet_sums_n(area, sh.nx, sh.ny, sh.nz, P.nf, sn);
                 if (tid == 0) {
-                    sh.scal[1] = s0;
-                    sh.scal[2] = s1;
-                    sh.scal[3] = s2;
+                    sh.scal[1] = sn[0];
+                    sh.scal[2] = sn[1];
+                    sh.scal[3] = sn[2];
                 }
             }
             __syncthreads();
