@@ -888,10 +888,18 @@ __device__ __forceinline__ void fit_one(const DevProblem &P, Shared2<NFM> &sh, c
             alamda = alamda_start;
             if (tid < ma) sh.a[tid] = sh.cg[tid];
         } else {
-            for (int it = tid; it < mfit * mfit; it += kWG2) {
-                const int j = it / mfit, k = it - j * mfit;
-                const double v = (cur ? sp.alpha1 : sp.alpha0)[it];
-                sh.u.g.cov[it] = (j == k) ? v * (1 + alamda) : v;
+            // the damped matrix: four reads in flight per lane; diagonal entries are every (mfit + 1)-th
+            const double *al = cur ? sp.alpha1 : sp.alpha0;
+            const int nn = mfit * mfit;
+            for (int i0 = tid; i0 < nn; i0 += 4 * kWG2) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = i0 + u * kWG2 < nn ? al[i0 + u * kWG2] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int it = i0 + u * kWG2;
+                    if (it < nn) sh.u.g.cov[it] = (it % (mfit + 1) == 0) ? v[u] * (1 + alamda) : v[u];
+                }
             }
             for (int j = tid; j < mfit; j += kWG2) sh.da[j] = (cur ? sp.beta1 : sp.beta0)[j];
             __syncthreads();
