@@ -119,9 +119,10 @@ __device__ inline void facet_sums_n(const double *area, const double *nx, const 
 
 // ---------------------------------------------------------------------------------------------
 // Phase B of one pass: the model rows of the wave's two tiles (pass-relative tiles 2w, 2w+1) into
-// sh.u.rows, and, in the last pass, the convexity rows.  The facet loop and epilogue arithmetic are
-// those of model_rows (ps_kernels.h), Dsph read through L2.  raw_y / raw_dyda (ps_eval) receive the
-// raw rows, all ma columns, in input order.
+// sh.u.rows, and, in the last pass, the convexity rows.  The facet arithmetic is that of model_rows
+// (ps_kernels.h), with Dsph read through L2, the two tiles' steps in lockstep, and the brightness taken
+// from the MFMA's coefficient-0 column instead of a sum of its own.  raw_y / raw_dyda (ps_eval) receive
+// the raw rows, all ma columns, in input order.
 // ---------------------------------------------------------------------------------------------
 template <int NT, int R, bool LS, int NFM>
 __device__ __forceinline__ void model_rows2(const DevProblem &P, Shared2<NFM> &sh, int tile0, int ntl, int rot,
