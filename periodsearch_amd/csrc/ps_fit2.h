@@ -777,7 +777,7 @@ __device__ inline int gauss_jordan2(Shared2<NFM> &sh, int n) {
             const double dum = read_lane(cur, icol);
             const double v = (piv_lane ? 0.0 : cur) - pr * dum;
             row[k] = v;
-            if (!((done >> lab[k]) & 1)) pivot_merge(c, {fabs(v), lab[k] * n + l}, cand_col);  // (uniform test)
+            if (!((done >> lab[k]) & 1)) pivot_merge(c, {fabs(v), lab[k] * n + l}, cand_col);  // a scalar branch: the row test is wave-uniform
         }
     }
 #pragma unroll
